@@ -1,0 +1,739 @@
+// HIP kernels of the MI355X DNS time step (gfx950, FP64 throughout).
+//
+// Nonlinear term pipeline for one field (reference: NSE::nonlinear,
+// channelflow/nse.cpp:383-391 -> diffops.cpp rotationalNL/convectionNL/
+// divergenceNL/skewsymmetricNL):
+//   k_yinv   Chebyshev coefficients -> values at the Chebyshev points y_n,
+//            with d/dy folded into the operator (T, T*D) and the base flow
+//            U(y), W(y), -Vsuck added to the kx=kz=0 mode;
+//   k_xinv   inverse x-FFT per (field, y, kz) line, optional d/dx;
+//   k_zpass  per (y, x) line: c2r z-FFTs, pointwise products, r2c z-FFTs;
+//   k_xfwd   forward x-FFT, live kx kept, 1/(Nx Nz) scaling;
+//   k_yfwd   values -> Chebyshev coefficients, with the spectral divergence
+//            of u_i u_j folded in for the divergence/skew-symmetric forms.
+// Only non-aliased modes |kx| <= kxd, kz <= kzd are ever stored, so the
+// reference's zeroPaddedModes() (flowfield.cpp:2235) is implicit.
+#include <hip/hip_runtime.h>
+
+#include "cfg_kernels.hpp"
+#include "cfg_math.hpp"
+
+namespace cfg {
+
+constexpr double PI = 3.14159265358979323846;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 imul(double s, double2 a) { return make_double2(-(s * a.y), s * a.x); }  // (i s) a
+
+__device__ __forceinline__ int kx_of(const Geom& g, int ikx) { return ikx <= g.kxd ? ikx : ikx - g.nkx; }
+
+// ---------------------------------------------------------------------------
+// LDS Stockham FFT over nlines contiguous lines of length P.N. sign -1 forward
+// (exp(-i...)), +1 backward; unnormalised. Returns the buffer holding the
+// result (a or b). All threads of the block must call it.
+// ---------------------------------------------------------------------------
+__device__ double2* fft_lds(double2* a, double2* b, const FFTPlan& P, int nlines, int sign) {
+    double2* src = a;
+    double2* dst = b;
+    const int N = P.N;
+    int Ns = 1;
+    for (int p = 0; p < P.npass; ++p) {
+        const int R = P.radix[p];
+        const int NR = N / R;
+        const int tstep = N / (Ns * R);
+        for (int w = threadIdx.x; w < nlines * NR; w += blockDim.x) {
+            const int line = w / NR;
+            const int j = w - line * NR;
+            const int k = j % Ns;
+            const double2* s = src + line * N;
+            double2 v[4];
+            for (int r = 0; r < R; ++r) v[r] = s[j + r * NR];
+            if (k != 0) {
+                for (int r = 1; r < R; ++r) {
+                    double2 t = P.tw[r * k * tstep];
+                    if (sign > 0) t.y = -t.y;
+                    v[r] = cmul(v[r], t);
+                }
+            }
+            double2 o[4];
+            if (R == 2) {
+                o[0] = cadd(v[0], v[1]);
+                o[1] = csub(v[0], v[1]);
+            } else if (R == 4) {
+                const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+                const double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+                // (-i * sign) * d13 ... forward: -i d13, backward: +i d13
+                const double2 jd = (sign < 0) ? make_double2(d13.y, -d13.x) : make_double2(-d13.y, d13.x);
+                o[0] = cadd(s02, s13);
+                o[1] = cadd(d02, jd);
+                o[2] = csub(s02, s13);
+                o[3] = csub(d02, jd);
+            } else {  // R == 3
+                const double s3 = (sign < 0) ? -0.86602540378443864676 : 0.86602540378443864676;
+                const double2 s12 = cadd(v[1], v[2]), d12 = csub(v[1], v[2]);
+                const double2 m = make_double2(v[0].x - 0.5 * s12.x, v[0].y - 0.5 * s12.y);
+                const double2 q = make_double2(-s3 * d12.y, s3 * d12.x);
+                o[0] = cadd(v[0], s12);
+                o[1] = cadd(m, q);
+                o[2] = csub(m, q);
+            }
+            double2* d = dst + line * N + (j / Ns) * Ns * R + k;
+            for (int r = 0; r < R; ++r) d[r * Ns] = o[r];
+        }
+        __syncthreads();
+        double2* t = src;
+        src = dst;
+        dst = t;
+        Ns *= R;
+    }
+    return src;
+}
+
+// ===========================================================================
+// Tau solver kernels: one thread per live mode (tausolver.cpp, nse.cpp:479)
+// ===========================================================================
+__device__ TauView make_view(const Geom& g, const TauSet& s, int m, double nu, bool taucorr) {
+    const int ikx = m % g.nkx, ikz = m / g.nkx;
+    const int kx = kx_of(g, ikx), kz = ikz;
+    const long S = g.M;
+    const double pi2 = PI * PI;
+    const double kappa2 = 4 * pi2 * ((kx / g.Lx) * (kx / g.Lx) + (kz / g.Lz) * (kz / g.Lz));
+    const double c = 4.0 * pi2 * nu;
+    const double lambda = s.lambda_t + c * ((kx / g.Lx) * (kx / g.Lx) + (kz / g.Lz) * (kz / g.Lz));
+    TauView t;
+    t.ph = HelmholtzView{SV{s.pband + m, S}, SV{s.pup + m, S}, SV{s.pinvd + m, S}, kappa2, g.Ny};
+    t.vh = HelmholtzView{SV{s.vband + m, S}, SV{s.vup + m, S}, SV{s.vinvd + m, S}, lambda, g.Ny};
+    t.Pp = SV{s.Pp + m, S};
+    t.Pm = SV{s.Pm + m, S};
+    t.vp = SV{s.vp + m, S};
+    t.vm = SV{s.vm + m, S};
+    t.P0 = SV{s.P0 + m, S};
+    t.v0 = SV{s.v0 + m, S};
+    t.scal = s.scal + m;
+    t.sstride = S;
+    t.nu = nu;
+    t.a = g.a;
+    t.b = g.b;
+    t.kx2pi = 2 * PI * kx / g.Lx;
+    t.kz2pi = 2 * PI * kz / g.Lz;
+    t.zero_mode = (kx == 0 && kz == 0);
+    t.taucorrection = taucorr;
+    return t;
+}
+
+__global__ void __launch_bounds__(64) k_tau_setup(Geom g, TauSet s, double nu, int taucorr, double* scratch) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.M) return;
+    const long S = g.M;
+    TauView t = make_view(g, s, m, nu, taucorr != 0);
+    SV s0{scratch + m, S}, s1{scratch + S * g.Ny + m, S};
+    tau_setup(t, s0, s1);
+}
+
+void launch_tau_setup(const Geom& g, const TauSet& s, double nu, bool taucorr, double* scratch, hipStream_t st) {
+    const int blk = 64;
+    k_tau_setup<<<(g.M + blk - 1) / blk, blk, 0, st>>>(g, s, nu, taucorr ? 1 : 0, scratch);
+}
+
+__device__ __forceinline__ CV cview(double* base, const Geom& g, int comp, int m) {
+    const long S2 = 2L * g.M;
+    double* p = base + 2L * ((long)comp * g.Ny * g.M + m);
+    return CV{SV{p, S2}, SV{p + 1, S2}};
+}
+
+__global__ void __launch_bounds__(64) k_tau_solve(Geom g, TauSet s, double nu, int taucorr, SolveConst c, double* rhs,
+                                                  double* u, double* q, double* scratch) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.M) return;
+    const long S = g.M;
+    TauView t = make_view(g, s, m, nu, taucorr != 0);
+    CV Rx = cview(rhs, g, 0, m), Ry = cview(rhs, g, 1, m), Rz = cview(rhs, g, 2, m);
+    CV U = cview(u, g, 0, m), V = cview(u, g, 1, m), W = cview(u, g, 2, m), P = cview(q, g, 0, m);
+    SV s0{scratch + m, S}, s1{scratch + S * g.Ny + m, S};
+    if (!t.zero_mode) {
+        tau_solve(t, U, V, W, P, Rx, Ry, Rz, s0, s1);
+        return;
+    }
+    // kx = kz = 0 (nse.cpp:505-547): base-flow dissipation and the mean
+    // pressure gradient or bulk-velocity constraint, then real parts only.
+    const int N = g.Ny;
+    for (int n = 0; n < N; ++n) Rx.re[n] += c.nuUbaseyy[n];
+    for (int n = 0; n < N; ++n) Rz.re[n] += c.nuWbaseyy[n];
+    if (!c.bulk) {
+        Rx.re[0] -= c.dPdxRef;
+        Rz.re[0] -= c.dPdzRef;
+        tau_solve(t, U, V, W, P, Rx, Ry, Rz, s0, s1);
+    } else {
+        SV s2{scratch + 2 * S * g.Ny + m, S}, s3{scratch + 3 * S * g.Ny + m, S};
+        double dPdx, dPdz;
+        tau_solve_bulk(t, U, V, W, P, Rx, Ry, Rz, c.umean, c.wmean, s0, s1, s2, s3, dPdx, dPdz);
+        *c.dPdx_out = dPdx;
+        *c.dPdz_out = dPdz;
+    }
+    for (int n = 0; n < N; ++n) {
+        U.im[n] = 0.0;
+        V.im[n] = 0.0;
+        W.im[n] = 0.0;
+        P.im[n] = 0.0;
+    }
+}
+
+void launch_tau_solve(const Geom& g, const TauSet& s, double nu, bool taucorr, const SolveConst& c, const double* rhs,
+                      double* u, double* q, double* scratch, hipStream_t st) {
+    const int blk = 64;
+    k_tau_solve<<<(g.M + blk - 1) / blk, blk, 0, st>>>(g, s, nu, taucorr ? 1 : 0, c, const_cast<double*>(rhs), u, q,
+                                                       scratch);
+}
+
+// ===========================================================================
+// Elementwise combination (dnsalgo.cpp:218-224, 668-673)
+// ===========================================================================
+struct CombineArgs {
+    double coef[8];
+    const double* in[8];
+};
+
+__global__ void k_combine(long n, int nterms, CombineArgs a, double* out) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int j = 0; j < nterms; ++j) acc += a.coef[j] * a.in[j][i];
+    out[i] = acc;
+}
+
+void launch_combine(long n, int nterms, const double* coef, const double* const* in, double* out, hipStream_t st) {
+    CombineArgs a;
+    for (int j = 0; j < 8; ++j) {
+        a.coef[j] = j < nterms ? coef[j] : 0.0;
+        a.in[j] = j < nterms ? in[j] : nullptr;
+    }
+    const int blk = 256;
+    k_combine<<<(int)((n + blk - 1) / blk), blk, 0, st>>>(n, nterms, a, out);
+}
+
+// ===========================================================================
+// NSE::linear on live modes (nse.cpp:393-477), one thread per mode
+// ===========================================================================
+__global__ void __launch_bounds__(64) k_linear(Geom g, double nu, LinearConst c, const double* u, const double* q,
+                                               double* out, double* scratch) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= g.M) return;
+    const int ikx = m % g.nkx, ikz = m / g.nkx;
+    const int kx = kx_of(g, ikx), kz = ikz;
+    const long S = g.M;
+    const int N = g.Ny;
+    const double L = g.b - g.a;
+    SV s0{scratch + m, S}, s1{scratch + S * N + m, S}, s2{scratch + 2 * S * N + m, S};
+    const double kappa2 = 4 * PI * PI * ((kx / g.Lx) * (kx / g.Lx) + (kz / g.Lz) * (kz / g.Lz));
+    const double dx = 2 * PI * kx / g.Lx;
+    const double dz = 2 * PI * kz / g.Lz;
+    CV P = cview(const_cast<double*>(q), g, 0, m);
+    double dPdxAct = 0.0, dPdzAct = 0.0;
+    for (int comp = 0; comp < 3; ++comp) {
+        CV U = cview(const_cast<double*>(u), g, comp, m);
+        CV O = cview(out, g, comp, m);
+        for (int part = 0; part < 2; ++part) {
+            SV up = part ? U.im : U.re;
+            for (int n = 0; n < N; ++n) s0[n] = nu * up[n];  // uk = nu u
+            cheb_diff(s0, s1, N, L);
+            if (kx == 0 && kz == 0 && c.bulk && part == 0 && comp != 1) {
+                const double eb = cheb_eval_b(s1, N), ea = cheb_eval_a(s1, N);
+                if (comp == 0) dPdxAct = (eb - ea) / L;
+                else dPdzAct = (eb - ea) / L;
+            }
+            cheb_diff(s1, s2, N, L);  // nu u''
+            SV o = part ? O.im : O.re;
+            for (int n = 0; n < N; ++n) o[n] = s2[n] - kappa2 * s0[n];
+        }
+        if (comp == 1) {  // - dq/dy
+            for (int part = 0; part < 2; ++part) {
+                SV pp = part ? P.im : P.re;
+                cheb_diff(pp, s1, N, L);
+                SV o = part ? O.im : O.re;
+                for (int n = 0; n < N; ++n) o[n] = o[n] - s1[n];
+            }
+        } else {  // - (i k) q
+            const double d = comp == 0 ? dx : dz;
+            for (int n = 0; n < N; ++n) {
+                // (0, d) * (Pr, Pi) = (-(d Pi), d Pr)
+                O.re[n] = O.re[n] - (0.0 * P.re[n] - d * P.im[n]);
+                O.im[n] = O.im[n] - (0.0 * P.im[n] + d * P.re[n]);
+            }
+        }
+    }
+    if (kx == 0 && kz == 0) {
+        CV O0 = cview(out, g, 0, m), O2 = cview(out, g, 2, m);
+        for (int n = 0; n < N; ++n) O0.re[n] += c.nuUbaseyy[n];
+        for (int n = 0; n < N; ++n) O2.re[n] += c.nuWbaseyy[n];
+        if (!c.bulk) {
+            O0.re[0] -= c.dPdxRef;
+            O2.re[0] -= c.dPdzRef;
+        } else {
+            dPdxAct += c.nuUbase_slope;
+            dPdzAct += c.nuWbase_slope;
+            O0.re[0] -= dPdxAct;
+            O2.re[0] -= dPdzAct;
+        }
+    }
+}
+
+void launch_linear(const Geom& g, double nu, const LinearConst& c, const double* u, const double* q, double* out,
+                   double* scratch, hipStream_t st) {
+    const int blk = 64;
+    k_linear<<<(g.M + blk - 1) / blk, blk, 0, st>>>(g, nu, c, u, q, out, scratch);
+}
+
+// ===========================================================================
+// Nonlinear term
+// ===========================================================================
+constexpr int YT = 8;  // y outputs per thread in the y-transforms
+
+// Chebyshev -> y values. Outputs per method:
+//   ROT : u v w wx wy wz      CONV/SKEW : u v w du/dy dv/dy dw/dy      DIV : u v w
+template <int METHOD>
+__global__ void __launch_bounds__(64) k_yinv(Geom g, YMats ym, const double2* __restrict__ u, double2* __restrict__ yp,
+                                             double Vsuck) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y0 = blockIdx.y * YT;
+    if (m >= g.M) return;
+    const int Ny = g.Ny;
+    const long M = g.M;
+    constexpr bool Q0 = (METHOD != NL_DIV), Q1 = (METHOD == NL_CONV || METHOD == NL_SKEW), Q2 = (METHOD != NL_DIV);
+    double2 P[3][YT], Q[3][YT];
+    for (int c = 0; c < 3; ++c)
+        for (int j = 0; j < YT; ++j) P[c][j] = Q[c][j] = make_double2(0.0, 0.0);
+    for (int k = 0; k < Ny; ++k) {
+        const double2 a0 = u[(0L * Ny + k) * M + m];
+        const double2 a1 = u[(1L * Ny + k) * M + m];
+        const double2 a2 = u[(2L * Ny + k) * M + m];
+#pragma unroll
+        for (int j = 0; j < YT; ++j) {
+            const int y = min(y0 + j, Ny - 1);
+            const double t = ym.T[(long)y * Ny + k];
+            P[0][j].x += t * a0.x; P[0][j].y += t * a0.y;
+            P[1][j].x += t * a1.x; P[1][j].y += t * a1.y;
+            P[2][j].x += t * a2.x; P[2][j].y += t * a2.y;
+            if (Q0 || Q1 || Q2) {
+                const double td = ym.TD[(long)y * Ny + k];
+                if (Q0) { Q[0][j].x += td * a0.x; Q[0][j].y += td * a0.y; }
+                if (Q1) { Q[1][j].x += td * a1.x; Q[1][j].y += td * a1.y; }
+                if (Q2) { Q[2][j].x += td * a2.x; Q[2][j].y += td * a2.y; }
+            }
+        }
+    }
+    const int ikx = m % g.nkx, ikz = m / g.nkx;
+    const double dx = 2 * PI * kx_of(g, ikx) / g.Lx, dz = 2 * PI * ikz / g.Lz;
+    for (int j = 0; j < YT; ++j) {
+        const int y = y0 + j;
+        if (y >= Ny) break;
+        if (m == 0) {  // base flow on the kx = kz = 0 mode (nse.cpp:28-36)
+            P[0][j].x += ym.base[0 * Ny + y];
+            Q[0][j].x += ym.base[1 * Ny + y];
+            P[2][j].x += ym.base[2 * Ny + y];
+            Q[2][j].x += ym.base[3 * Ny + y];
+            P[1][j].x -= Vsuck;
+        }
+        double2 o[6];
+        o[0] = P[0][j];
+        o[1] = P[1][j];
+        o[2] = P[2][j];
+        int nout = 3;
+        if (METHOD == NL_ROT) {
+            o[3] = csub(Q[2][j], imul(dz, P[1][j]));                // dw/dy - dv/dz
+            o[4] = csub(imul(dz, P[0][j]), imul(dx, P[2][j]));      // du/dz - dw/dx
+            o[5] = csub(imul(dx, P[1][j]), Q[0][j]);                // dv/dx - du/dy
+            nout = 6;
+        } else if (METHOD == NL_CONV || METHOD == NL_SKEW) {
+            o[3] = Q[0][j];
+            o[4] = Q[1][j];
+            o[5] = Q[2][j];
+            nout = 6;
+        }
+        for (int f = 0; f < nout; ++f) yp[((long)f * Ny + y) * M + m] = o[f];
+    }
+}
+
+// inverse x-FFT: yp [F][Ny][nkz][nkx] -> xp [Fo][Ny][Nx][nkz]; output f >= nsrc is
+// d/dx of source field f - nsrc
+constexpr int XLB = 8;  // lines (kz values) per block in the x-FFT kernels
+
+__global__ void __launch_bounds__(256) k_xinv(Geom g, FFTPlan px, int nsrc, const double2* __restrict__ yp,
+                                              double2* __restrict__ xp) {
+    extern __shared__ double2 lds[];
+    const int Nx = g.Nx, nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
+    const int nchunk = (nkz + XLB - 1) / XLB;
+    const int y = blockIdx.x / nchunk, ikz0 = (blockIdx.x % nchunk) * XLB;
+    const int fo = blockIdx.y;
+    const bool deriv = fo >= nsrc;
+    const int fs = deriv ? fo - nsrc : fo;
+    const int nl = min(XLB, nkz - ikz0);
+    double2* a = lds;
+    double2* b = lds + XLB * Nx;
+    for (int w = threadIdx.x; w < nl * Nx; w += blockDim.x) {
+        const int l = w / Nx, j = w - l * Nx;
+        int ikx = -1;
+        if (j <= g.kxd) ikx = j;
+        else if (j >= Nx - g.kxd) ikx = j - Nx + nkx;
+        double2 v = make_double2(0.0, 0.0);
+        if (ikx >= 0) {
+            v = yp[(((long)fs * Ny + y) * nkz + ikz0 + l) * nkx + ikx];
+            if (deriv) v = imul(2 * PI * kx_of(g, ikx) / g.Lx, v);
+        }
+        a[l * Nx + j] = v;
+    }
+    __syncthreads();
+    double2* r = fft_lds(a, b, px, nl, +1);
+    for (int w = threadIdx.x; w < nl * Nx; w += blockDim.x) {
+        const int x = w / nl, l = w - x * nl;
+        xp[(((long)fo * Ny + y) * Nx + x) * nkz + ikz0 + l] = r[l * Nx + x];
+    }
+}
+
+// forward x-FFT: zo [G][Ny][Nx][nkz] -> fo [G][Ny][nkz][nkx], scaled by 1/(Nx Nz)
+__global__ void __launch_bounds__(256) k_xfwd(Geom g, FFTPlan px, const double2* __restrict__ zo,
+                                              double2* __restrict__ fo) {
+    extern __shared__ double2 lds[];
+    const int Nx = g.Nx, nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
+    const int nchunk = (nkz + XLB - 1) / XLB;
+    const int y = blockIdx.x / nchunk, ikz0 = (blockIdx.x % nchunk) * XLB;
+    const int f = blockIdx.y;
+    const int nl = min(XLB, nkz - ikz0);
+    double2* a = lds;
+    double2* b = lds + XLB * Nx;
+    for (int w = threadIdx.x; w < nl * Nx; w += blockDim.x) {
+        const int x = w / nl, l = w - x * nl;
+        a[l * Nx + x] = zo[(((long)f * Ny + y) * Nx + x) * nkz + ikz0 + l];
+    }
+    __syncthreads();
+    double2* r = fft_lds(a, b, px, nl, -1);
+    const double scale = 1.0 / ((double)g.Nx * g.Nz);
+    for (int w = threadIdx.x; w < nl * nkx; w += blockDim.x) {
+        const int l = w / nkx, ikx = w - l * nkx;
+        const int j = ikx <= g.kxd ? ikx : ikx - nkx + Nx;
+        double2 v = r[l * Nx + j];
+        fo[(((long)f * Ny + y) * nkz + ikz0 + l) * nkx + ikx] = make_double2(v.x * scale, v.y * scale);
+    }
+}
+
+// ---- z pass --------------------------------------------------------------
+struct ZCtx {
+    Geom g;
+    FFTPlan pz;
+    int y, x0, nl;
+    double2 *wa, *wb;
+    double* phys;  // [NP][XB][Nz]
+    int XB;
+};
+
+__device__ __forceinline__ double* physf(const ZCtx& c, int f) { return c.phys + (long)f * c.XB * c.g.Nz; }
+
+// c2r of two real fields at once: field fa -> phys[da], fb -> phys[db]
+// (fb < 0: none). dza/dzb apply d/dz. Imag parts of kz = 0 are ignored, as
+// FFTW's c2r does.
+__device__ void zc2r_pair(const ZCtx& c, const double2* __restrict__ xp, int fa, bool dza, int da, int fb, bool dzb,
+                          int db) {
+    const Geom& g = c.g;
+    const int Nz = g.Nz, nkz = g.nkz;
+    for (int w = threadIdx.x; w < c.nl * Nz; w += blockDim.x) c.wa[w] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int w = threadIdx.x; w < c.nl * nkz; w += blockDim.x) {
+        const int l = w / nkz, k = w - l * nkz;
+        const long base = ((long)c.y * g.Nx + c.x0 + l) * nkz + k;
+        const double s = 2 * PI * k / g.Lz;
+        double2 A = xp[(long)fa * g.Ny * g.Nx * nkz + base];
+        if (dza) A = imul(s, A);
+        double2 B = make_double2(0.0, 0.0);
+        if (fb >= 0) {
+            B = xp[(long)fb * g.Ny * g.Nx * nkz + base];
+            if (dzb) B = imul(s, B);
+        }
+        double2* z = c.wa + l * Nz;
+        if (k == 0) {
+            z[0] = make_double2(A.x, B.x);
+        } else {
+            z[k] = make_double2(A.x - B.y, A.y + B.x);         // A + iB
+            z[Nz - k] = make_double2(A.x + B.y, B.x - A.y);    // conj(A) + i conj(B)
+        }
+    }
+    __syncthreads();
+    double2* r = fft_lds(c.wa, c.wb, c.pz, c.nl, +1);
+    double* pa = physf(c, da);
+    double* pb = db >= 0 ? physf(c, db) : nullptr;
+    for (int w = threadIdx.x; w < c.nl * Nz; w += blockDim.x) {
+        pa[w] = r[w].x;
+        if (pb) pb[w] = r[w].y;
+    }
+    __syncthreads();
+}
+
+// r2c of phys[sa] (and phys[sb]) -> zo fields ga, gb (kz <= kzd kept)
+__device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int ga, int sb, int gb) {
+    const Geom& g = c.g;
+    const int Nz = g.Nz, nkz = g.nkz;
+    const double* pa = physf(c, sa);
+    const double* pb = sb >= 0 ? physf(c, sb) : nullptr;
+    for (int w = threadIdx.x; w < c.nl * Nz; w += blockDim.x) c.wa[w] = make_double2(pa[w], pb ? pb[w] : 0.0);
+    __syncthreads();
+    double2* r = fft_lds(c.wa, c.wb, c.pz, c.nl, -1);
+    for (int w = threadIdx.x; w < c.nl * nkz; w += blockDim.x) {
+        const int l = w / nkz, k = w - l * nkz;
+        const double2 Zk = r[l * Nz + k];
+        const double2 Zn = r[l * Nz + ((Nz - k) % Nz)];
+        const long base = ((long)c.y * g.Nx + c.x0 + l) * nkz + k;
+        zo[(long)ga * g.Ny * g.Nx * nkz + base] = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
+        if (gb >= 0)
+            zo[(long)gb * g.Ny * g.Nx * nkz + base] = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
+    }
+    __syncthreads();
+}
+
+// physical-space products. xp fields: ROT u v w wx wy wz; CONV/SKEW u v w
+// du/dy dv/dy dw/dy du/dx dv/dx dw/dx; DIV u v w. zo outputs: g0..g2 (the
+// u.grad u / omega x u part and the rotation term) then uu00 uu01 uu02 uu11
+// uu12 uu22 for DIV/SKEW.
+template <int METHOD>
+__global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, double rotation, int want_g,
+                                               const double2* __restrict__ xp, double2* __restrict__ zo) {
+    extern __shared__ double2 lds[];
+    const int nxc = (g.Nx + XB - 1) / XB;
+    ZCtx c;
+    c.g = g;
+    c.pz = pz;
+    c.y = blockIdx.x / nxc;
+    c.x0 = (blockIdx.x % nxc) * XB;
+    c.nl = min(XB, g.Nx - c.x0);
+    c.XB = XB;
+    c.wa = lds;
+    c.wb = lds + XB * g.Nz;
+    c.phys = reinterpret_cast<double*>(lds + 2 * XB * g.Nz);
+    const int Nz = g.Nz;
+    const int npts = c.nl * Nz;
+    if (METHOD == NL_ROT) {
+        zc2r_pair(c, xp, 0, false, 0, 1, false, 1);
+        zc2r_pair(c, xp, 2, false, 2, 3, false, 3);
+        zc2r_pair(c, xp, 4, false, 4, 5, false, 5);
+        double *u0 = physf(c, 0), *u1 = physf(c, 1), *u2 = physf(c, 2);
+        double *w0 = physf(c, 3), *w1 = physf(c, 4), *w2 = physf(c, 5);
+        for (int w = threadIdx.x; w < npts; w += blockDim.x) {  // cross(vort, u) (diffops.cpp:2598-2605)
+            const double a0 = u0[w], a1 = u1[w], a2 = u2[w], b0 = w0[w], b1 = w1[w], b2 = w2[w];
+            double f0 = b1 * a2 - b2 * a1;
+            double f1 = b2 * a0 - b0 * a2;
+            const double f2 = b0 * a1 - b1 * a0;
+            if (rotation != 0.0) {  // nse.cpp:61-80
+                f0 -= rotation * a1;
+                f1 += rotation * a0;
+            }
+            w0[w] = f0;
+            w1[w] = f1;
+            w2[w] = f2;
+        }
+        __syncthreads();
+        zr2c_pair(c, zo, 3, 0, 4, 1);
+        zr2c_pair(c, zo, 5, 2, -1, -1);
+        return;
+    }
+    // u, v, w -> phys 0..2
+    zc2r_pair(c, xp, 0, false, 0, 1, false, 1);
+    zc2r_pair(c, xp, 2, false, 2, -1, false, -1);
+    if (METHOD == NL_CONV || METHOD == NL_SKEW) {
+        // f_i = sum_j (s u_j) du_i/dx_j accumulated over j = x, y, z
+        // (diffops.cpp:3192-3199, 3626-3633); phys 3..5 = f, 6..8 = du_i/dx_j
+        const double s = (METHOD == NL_SKEW) ? 0.5 : 1.0;
+        double *u0 = physf(c, 0), *u1 = physf(c, 1), *u2 = physf(c, 2);
+        double *f0 = physf(c, 3), *f1 = physf(c, 4), *f2 = physf(c, 5);
+        double *t0 = physf(c, 6), *t1 = physf(c, 7), *t2 = physf(c, 8);
+        for (int j = 0; j < 3; ++j) {
+            if (j == 0) {  // d/dx from xp 6..8
+                zc2r_pair(c, xp, 6, false, 6, 7, false, 7);
+                zc2r_pair(c, xp, 8, false, 8, -1, false, -1);
+            } else if (j == 1) {  // d/dy from xp 3..5
+                zc2r_pair(c, xp, 3, false, 6, 4, false, 7);
+                zc2r_pair(c, xp, 5, false, 8, -1, false, -1);
+            } else {  // d/dz of u v w
+                zc2r_pair(c, xp, 0, true, 6, 1, true, 7);
+                zc2r_pair(c, xp, 2, true, 8, -1, false, -1);
+            }
+            const double* uj = (j == 0) ? u0 : (j == 1) ? u1 : u2;
+            for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+                const double su = (METHOD == NL_SKEW) ? s * uj[w] : uj[w];
+                if (j == 0) {
+                    f0[w] = 0.0 + su * t0[w];
+                    f1[w] = 0.0 + su * t1[w];
+                    f2[w] = 0.0 + su * t2[w];
+                } else {
+                    f0[w] += su * t0[w];
+                    f1[w] += su * t1[w];
+                    f2[w] += su * t2[w];
+                }
+            }
+            __syncthreads();
+        }
+        if (rotation != 0.0) {
+            for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+                f0[w] -= rotation * u1[w];
+                f1[w] += rotation * u0[w];
+            }
+            __syncthreads();
+        }
+        zr2c_pair(c, zo, 3, 0, 4, 1);
+        zr2c_pair(c, zo, 5, 2, -1, -1);
+    } else if (want_g) {  // DIV with rotation: g = rotation terms only
+        double *u0 = physf(c, 0), *u1 = physf(c, 1);
+        double *f0 = physf(c, 3), *f1 = physf(c, 4), *f2 = physf(c, 5);
+        for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+            f0[w] = -(rotation * u1[w]);
+            f1[w] = rotation * u0[w];
+            f2[w] = 0.0;
+        }
+        __syncthreads();
+        zr2c_pair(c, zo, 3, 0, 4, 1);
+        zr2c_pair(c, zo, 5, 2, -1, -1);
+    }
+    if (METHOD == NL_DIV || METHOD == NL_SKEW) {  // outer(u, u) (diffops.cpp:2336-2388)
+        double *u0 = physf(c, 0), *u1 = physf(c, 1), *u2 = physf(c, 2);
+        double *p3 = physf(c, 3), *p4 = physf(c, 4), *p5 = physf(c, 5);
+        double *p6 = physf(c, 6), *p7 = physf(c, 7), *p8 = physf(c, 8);
+        for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+            const double a0 = u0[w], a1 = u1[w], a2 = u2[w];
+            p3[w] = a0 * a0;
+            p4[w] = a0 * a1;
+            p5[w] = a0 * a2;
+            p6[w] = a1 * a1;
+            p7[w] = a1 * a2;
+            p8[w] = a2 * a2;
+        }
+        __syncthreads();
+        zr2c_pair(c, zo, 3, 3, 4, 4);
+        zr2c_pair(c, zo, 5, 5, 6, 6);
+        zr2c_pair(c, zo, 7, 7, 8, 8);
+    }
+}
+
+// y values -> Chebyshev coefficients, f_i = Ti G_i + DTi H_i with
+//   ROT/CONV : G_i = g_i,                                    H_i = 0
+//   SKEW     : G_i = g_i + 0.5 (ikx uu_i0 + ikz uu_i2),        H_i = 0.5 uu_i1
+//   DIV      : G_i = [g_i] + (ikx uu_i0 + ikz uu_i2),          H_i = uu_i1
+template <int METHOD>
+__global__ void __launch_bounds__(64) k_yfwd(Geom g, YMats ym, int have_g, const double2* __restrict__ fo,
+                                             double2* __restrict__ f) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k0 = blockIdx.y * YT;
+    if (m >= g.M) return;
+    const int Ny = g.Ny;
+    const long M = g.M;
+    const int ikx = m % g.nkx, ikz = m / g.nkx;
+    const double dx = 2 * PI * kx_of(g, ikx) / g.Lx, dz = 2 * PI * ikz / g.Lz;
+    constexpr bool DIVF = (METHOD == NL_DIV || METHOD == NL_SKEW);
+    const double beta = (METHOD == NL_SKEW) ? 0.5 : 1.0;
+    // symmetric uu index -> fo field: 00:3 01:4 02:5 11:6 12:7 22:8
+    const int uuf[3][3] = {{3, 4, 5}, {4, 6, 7}, {5, 7, 8}};
+    double2 A[3][YT];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < YT; ++j) A[i][j] = make_double2(0.0, 0.0);
+    for (int y = 0; y < Ny; ++y) {
+        double2 G[3], H[3];
+        for (int i = 0; i < 3; ++i) {
+            G[i] = have_g ? fo[((long)i * Ny + y) * M + m] : make_double2(0.0, 0.0);
+            H[i] = make_double2(0.0, 0.0);
+            if (DIVF) {
+                const double2 x0 = fo[((long)uuf[i][0] * Ny + y) * M + m];
+                const double2 x1 = fo[((long)uuf[i][1] * Ny + y) * M + m];
+                const double2 x2 = fo[((long)uuf[i][2] * Ny + y) * M + m];
+                const double2 d = cadd(imul(dx, x0), imul(dz, x2));
+                G[i] = make_double2(G[i].x + beta * d.x, G[i].y + beta * d.y);
+                H[i] = make_double2(beta * x1.x, beta * x1.y);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < YT; ++j) {
+            const int k = min(k0 + j, Ny - 1);
+            const double t = ym.Ti[(long)k * Ny + y];
+            for (int i = 0; i < 3; ++i) {
+                A[i][j].x += t * G[i].x;
+                A[i][j].y += t * G[i].y;
+            }
+            if (DIVF) {
+                const double td = ym.DTi[(long)k * Ny + y];
+                for (int i = 0; i < 3; ++i) {
+                    A[i][j].x += td * H[i].x;
+                    A[i][j].y += td * H[i].y;
+                }
+            }
+        }
+    }
+    for (int j = 0; j < YT; ++j) {
+        const int k = k0 + j;
+        if (k >= Ny) break;
+        for (int i = 0; i < 3; ++i) f[((long)i * Ny + k) * M + m] = A[i][j];
+    }
+}
+
+static int zpass_xb(const Geom& g, int nphys) {
+    const long per = (long)g.Nz * (32 + 8 * nphys);  // two complex work lines + nphys real lines
+    int xb = 1;
+    while (xb < 32 && xb * 2 <= g.Nx && (long)(xb * 2) * per <= 64 * 1024) xb *= 2;
+    return xb;
+}
+
+void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, double Vsuck,
+                      double rotation, const double* u, double* f, const NLWork& w, hipStream_t st) {
+    const dim3 ygrid((g.M + 63) / 64, (g.Ny + YT - 1) / YT);
+    const double2* u2 = reinterpret_cast<const double2*>(u);
+    double2* yp = reinterpret_cast<double2*>(w.yp);
+    double2* xp = reinterpret_cast<double2*>(w.xp);
+    double2* zo = reinterpret_cast<double2*>(w.zo);
+    double2* fo = reinterpret_cast<double2*>(w.fo);
+    double2* f2 = reinterpret_cast<double2*>(f);
+    const int nchunk = (g.nkz + XLB - 1) / XLB;
+    const size_t xlds = 2 * XLB * g.Nx * sizeof(double2);
+    int nsrc = 3, nxout = 3;
+    switch (method) {
+        case NL_ROT:
+            k_yinv<NL_ROT><<<ygrid, 64, 0, st>>>(g, ym, u2, yp, Vsuck);
+            nsrc = 6; nxout = 6;
+            break;
+        case NL_CONV:
+            k_yinv<NL_CONV><<<ygrid, 64, 0, st>>>(g, ym, u2, yp, Vsuck);
+            nsrc = 6; nxout = 9;
+            break;
+        case NL_SKEW:
+            k_yinv<NL_SKEW><<<ygrid, 64, 0, st>>>(g, ym, u2, yp, Vsuck);
+            nsrc = 6; nxout = 9;
+            break;
+        default:
+            k_yinv<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, u2, yp, Vsuck);
+            nsrc = 3; nxout = 3;
+    }
+    // d/dx outputs (fields nsrc..) derive from fields 0..2 only
+    k_xinv<<<dim3(g.Ny * nchunk, nxout), 256, xlds, st>>>(g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, yp, xp);
+    const int nphys = (method == NL_ROT) ? 6 : 9;
+    const int XB = zpass_xb(g, nphys);
+    const size_t zlds = (size_t)XB * g.Nz * (2 * sizeof(double2) + nphys * sizeof(double));
+    const int zgrid = g.Ny * ((g.Nx + XB - 1) / XB);
+    const int want_g = (method != NL_DIV) || (rotation != 0.0);
+    switch (method) {
+        case NL_ROT: k_zpass<NL_ROT><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo); break;
+        case NL_CONV: k_zpass<NL_CONV><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo); break;
+        case NL_SKEW: k_zpass<NL_SKEW><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo); break;
+        default: k_zpass<NL_DIV><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo);
+    }
+    // forward x-FFT of the z-pass outputs that exist
+    if (want_g) k_xfwd<<<dim3(g.Ny * nchunk, 3), 256, xlds, st>>>(g, px, zo, fo);
+    if (method == NL_DIV || method == NL_SKEW) {
+        const long off = 3L * g.Ny * g.Nx * g.nkz;
+        const long offo = 3L * g.Ny * g.M;
+        k_xfwd<<<dim3(g.Ny * nchunk, 6), 256, xlds, st>>>(g, px, zo + off, fo + offo);
+    }
+    switch (method) {
+        case NL_ROT: k_yfwd<NL_ROT><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, f2); break;
+        case NL_CONV: k_yfwd<NL_CONV><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, f2); break;
+        case NL_SKEW: k_yfwd<NL_SKEW><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, f2); break;
+        default: k_yfwd<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, want_g, fo, f2);
+    }
+}
+
+}  // namespace cfg

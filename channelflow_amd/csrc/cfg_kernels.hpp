@@ -1,0 +1,83 @@
+// Kernel launchers of the MI355X DNS time step (implemented in cfg_kernels.hip).
+// All buffers are device pointers; all arithmetic is FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cfg {
+
+// Compact spectral layout of one field with nd components:
+//   complex element (c, ny, ikz, ikx) at index ((c*Ny + ny)*nkz + ikz)*nkx + ikx,
+//   kx = ikx for ikx <= kxd, ikx - nkx otherwise; kz = ikz.
+// Only live modes |kx| <= kxd, kz <= kzd are stored (M = nkx*nkz per plane).
+struct Geom {
+    int Nx, Ny, Nz;
+    int kxd, kzd, nkx, nkz, M;
+    double Lx, Lz, a, b;
+};
+
+struct FFTPlan {
+    int N;
+    int npass;
+    int radix[12];
+    const double2* tw;  // exp(-2 pi i j / N), j < N
+};
+
+enum NLMethod { NL_ROT = 0, NL_CONV = 1, NL_DIV = 2, NL_SKEW = 3 };
+
+struct YMats {            // y-direction operators, row-major [row][col], Ny x Ny
+    const double* T;      // Chebyshev coefficients -> values at y_n
+    const double* TD;     // T * (d/dy)
+    const double* Ti;     // values -> coefficients
+    const double* DTi;    // (d/dy) * Ti
+    const double* base;   // [4][Ny]: U(y), dU/dy(y), W(y), dW/dy(y) at the y_n
+};
+
+struct TauSet {  // per-mode tau data for one lambda_t, each array [n][M] (scal: [6][M])
+    double *pband, *pup, *pinvd, *vband, *vup, *vinvd;
+    double *Pp, *Pm, *vp, *vm, *P0, *v0;
+    double* scal;
+    double lambda_t;
+};
+
+struct SolveConst {  // kx=kz=0 mode data (nse.cpp:505-547)
+    const double* nuUbaseyy;  // [Ny] nu * U''(coefficients)
+    const double* nuWbaseyy;
+    double dPdxRef, dPdzRef;
+    double umean, wmean;  // UbulkRef - UbulkBase, WbulkRef - WbulkBase
+    int bulk;             // 1: BulkVelocity constraint
+    double* dPdx_out;     // device scalars written by the bulk solve
+    double* dPdz_out;
+};
+
+struct LinearConst {  // kx=kz=0 constants of NSE::linear (nse.cpp:437-472)
+    const double* nuUbaseyy;
+    const double* nuWbaseyy;
+    double dPdxRef, dPdzRef;
+    int bulk;
+    double nuUbase_slope, nuWbase_slope;  // nu (U'(b) - U'(a)) / Ly etc.
+};
+
+// ---- tau ------------------------------------------------------------------
+void launch_tau_setup(const Geom& g, const TauSet& s, double nu, bool taucorr, double* scratch, hipStream_t st);
+void launch_tau_solve(const Geom& g, const TauSet& s, double nu, bool taucorr, const SolveConst& c,
+                      const double* rhs, double* u, double* q, double* scratch, hipStream_t st);
+
+// ---- elementwise ----------------------------------------------------------
+// out = sum_j coef[j] * in[j], over n doubles (up to 8 terms)
+void launch_combine(long n, int nterms, const double* coef, const double* const* in, double* out, hipStream_t st);
+// NSE::linear on live modes: out = nu (u'' - kappa^2 u) - grad q (+ constants)
+void launch_linear(const Geom& g, double nu, const LinearConst& c, const double* u, const double* q, double* out,
+                   double* scratch, hipStream_t st);
+
+// ---- nonlinear term ---------------------------------------------------------
+struct NLWork {
+    double* yp;  // [6][Ny][M] complex
+    double* xp;  // [9][Ny][Nx][nkz] complex
+    double* zo;  // [9][Ny][Nx][nkz] complex
+    double* fo;  // [9][Ny][M] complex
+};
+void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method,
+                      double Vsuck, double rotation, const double* u, double* f, const NLWork& w, hipStream_t st);
+
+}  // namespace cfg

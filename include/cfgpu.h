@@ -1,0 +1,108 @@
+/* cfgpu -- C ABI of the MI355X-native channelflow DNS time step.
+ *
+ * This is the drop-in boundary for channelflow's DNS::advance path. Host code
+ * keeps channelflow's FlowField / DNS / DNSFlags interface (see
+ * include/cfgpu_dns.hpp for the C++ facade) and calls the GPU through these
+ * entry points only: plain pointers and sizes, integer status codes, no
+ * exit() inside the library.
+ *
+ * Spectral field buffers passed across this boundary use channelflow's serial
+ * complex layout (channelflow/flowfield.h:387-401), interleaved re/im doubles:
+ *     element (mx, my, mz, i) at complex index  mz + (Nz/2+1)*(mx + Nx*(my + Ny*i))
+ * with kx = mx for mx <= Nx/2, mx - Nx otherwise, and kz = mz.
+ * Modes outside the live set (|kx| > kxd, kz > kzd, and the Nyquist modes) are
+ * ignored on input and returned as zero.
+ *
+ * Status codes: 0 success, <0 failure; cfg_last_error() describes the last
+ * failure on the calling thread.
+ */
+#ifndef CFGPU_H
+#define CFGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFG_OK 0
+#define CFG_ERR_ARG -1       /* invalid argument / unsupported option     */
+#define CFG_ERR_DEVICE -2    /* HIP runtime failure                        */
+#define CFG_ERR_NONFINITE -3 /* non-finite value detected in the state     */
+#define CFG_ERR_STATE -4     /* call not valid in the object's state       */
+
+/* Enum values follow channelflow/dnsflags.h:26-38. */
+enum cfg_baseflow { CFG_ZeroBase = 0, CFG_LinearBase, CFG_ParabolicBase, CFG_LaminarBase, CFG_SuctionBase, CFG_ArbitraryBase };
+enum cfg_constraint { CFG_PressureGradient = 0, CFG_BulkVelocity };
+enum cfg_stepmethod { CFG_CNFE1 = 0, CFG_CNAB2, CFG_CNRK2, CFG_SMRK2, CFG_SBDF1, CFG_SBDF2, CFG_SBDF3, CFG_SBDF4 };
+enum cfg_nonlinearity { CFG_Rotational = 0, CFG_Convection, CFG_Divergence, CFG_SkewSymmetric, CFG_Alternating, CFG_Alternating_ };
+enum cfg_dealiasing { CFG_NoDealiasing = 0, CFG_DealiasXZ, CFG_DealiasY, CFG_DealiasXYZ };
+
+/* The DNSFlags fields read by DNS::advance (channelflow/dnsflags.h:106-140). */
+typedef struct cfg_flags {
+    int baseflow, constraint, timestepping, initstepping, nonlinearity, dealiasing, taucorrection, reserved;
+    double nu, Vsuck, rotation, dPdx, dPdz, Ubulk, Wbulk;
+    double ulowerwall, uupperwall, wlowerwall, wupperwall, t0, dt;
+} cfg_flags;
+
+/* Grid of a FlowField(Nx, Ny, Nz, Nd, Lx, Lz, a, b) (channelflow/flowfield.h:56). */
+typedef struct cfg_grid {
+    int Nx, Ny, Nz, reserved;
+    double Lx, Lz, a, b;
+} cfg_grid;
+
+typedef struct cfg_dns cfg_dns;
+
+const char* cfg_last_error(void);
+int cfg_device_count(int* n);
+
+/* DNS(fields, flags) / DNS(fields, {Ubase, Wbase}, flags) -- channelflow/dns.h:34-35,
+ * dns.cpp:22-62. u_spec: 3-component velocity, q_spec: 1-component pressure
+ * (may be NULL for zero). Ubase/Wbase: Chebyshev coefficients (length Ny) of an
+ * arbitrary base flow, or NULL to build it from flags.baseflow. */
+int cfg_dns_create(const cfg_grid* grid, const cfg_flags* flags, const double* u_spec, const double* q_spec,
+                   const double* Ubase, const double* Wbase, cfg_dns** out);
+int cfg_dns_destroy(cfg_dns* dns);
+
+/* DNS::advance(fields, nSteps) -- dns.cpp:135-163. The velocity and pressure
+ * stay resident on the device between calls. */
+int cfg_dns_advance(cfg_dns* dns, int nsteps);
+
+/* The caller's `fields` of DNS::advance: set replaces u, q as a caller
+ * would between advance() calls; get copies them out. */
+int cfg_dns_set_state(cfg_dns* dns, const double* u_spec, const double* q_spec);
+int cfg_dns_get_state(cfg_dns* dns, double* u_spec, double* q_spec);
+
+/* DNS::reset_dt / reset_time / reset_gradp / reset_bulkv (dns.cpp:180-232). */
+int cfg_dns_reset_dt(cfg_dns* dns, double dt);
+int cfg_dns_reset_time(cfg_dns* dns, double t);
+int cfg_dns_reset_gradp(cfg_dns* dns, double dPdx, double dPdz);
+int cfg_dns_reset_bulkv(cfg_dns* dns, double Ubulk, double Wbulk);
+
+/* DNS::time, dPdx, Ubulk, dt, CFL(u) (dns.cpp:263-330, dnsalgo.cpp:63-67). */
+int cfg_dns_time(const cfg_dns* dns, double* t);
+int cfg_dns_dpdx(cfg_dns* dns, double* dPdx);
+int cfg_dns_ubulk(const cfg_dns* dns, double* Ubulk);
+int cfg_dns_cfl(cfg_dns* dns, double* cfl);
+/* Ubase(), Wbase() (dns.h:77-78): Chebyshev coefficients, length Ny. */
+int cfg_dns_base(const cfg_dns* dns, double* Ubase, double* Wbase);
+
+/* Device pointers to the resident velocity and pressure, valid until the next
+ * advance. Compact live-mode layout: complex element (c, ny, ikz, ikx) at
+ * ((c*Ny + ny)*nkz + ikz)*nkx + ikx, kx = ikx for ikx <= (nkx-1)/2 and
+ * ikx - nkx otherwise, kz = ikz. */
+int cfg_dns_device_state(cfg_dns* dns, double** u_dev, double** q_dev, int* nkx, int* nkz);
+
+/* One NSE::nonlinear evaluation (nse.cpp:383-391): base flow added,
+ * nonlinearity per flags.nonlinearity, aliased modes of f zero. */
+int cfg_nonlinear(const cfg_grid* grid, const cfg_flags* flags, const double* u_spec, double* f_spec);
+
+/* One NSE::solve with TauSolvers built for lambda_t (nse.cpp:479-575,
+ * tausolver.cpp:347-450): rhs -> (u, q) on live modes. u_spec/q_spec are
+ * overwritten on live modes only; dPdx (may be NULL) receives the bulk-
+ * velocity pressure gradient when flags.constraint is BulkVelocity. */
+int cfg_solve(const cfg_grid* grid, const cfg_flags* flags, double lambda_t, const double* u0_spec,
+              const double* rhs_spec, double* u_spec, double* q_spec, double* dPdx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CFGPU_H */

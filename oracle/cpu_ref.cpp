@@ -853,7 +853,7 @@ void div(FlowField& f, FlowField& divf, FieldState finalstate) {  // diffops.cpp
             }
     }
     f.makeState(sxz, sy);
-    if (finalstate == Physical) divf.makePhysical();
+    if (finalstate == Spectral) divf.makeSpectral();  // diffops.cpp:2554-2556: no Physical branch
 }
 
 void rotationalNL(FlowField& u, FlowField& f, FlowField& vort, FieldState finalstate) {  // diffops.cpp:2852-2881

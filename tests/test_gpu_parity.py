@@ -1,0 +1,144 @@
+"""GPU parity: the HIP path through the C ABI against the oracle (oracle/cpu_ref.cpp).
+
+Tolerances (north star): relative L2 <= 1e-12 per velocity component for one
+nonlinear-term or one tau-solve call; <= 1e-10 after 20 time steps; the
+reference's own golden trajectory (tests/timeIntegrationTest.cpp) at its own
+bar, L2Dist <= 1e-13.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cfref as R
+import channelflow_amd as cfa
+from channelflow_amd.fields import random_field
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LX_C = 2 * np.pi / 1.14
+LZ_C = 2 * np.pi / 2.5
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2_per_component(g, a, b):
+    """L2Dist(a_i, b_i) / L2Norm(b_i) per component (diffops.cpp:353-487 norms)."""
+    out = []
+    for i in range(a.shape[0]):
+        d = np.sqrt(R.l2dist2(g, a[i:i + 1], b[i:i + 1]))
+        n = np.sqrt(R.l2norm2(g, b[i:i + 1]))
+        out.append(d / n if n > 0 else d)
+    return np.array(out)
+
+
+def live_mask(Nx, Nz, kxd, kzd):
+    kx = np.fft.fftfreq(Nx, 1.0 / Nx).astype(int)
+    m = np.zeros((Nx, Nz // 2 + 1), dtype=bool)
+    m[np.abs(kx) <= kxd, :kzd + 1] = True
+    return m
+
+
+@pytest.mark.parametrize("method", ["Rotational", "Convection", "Divergence", "SkewSymmetric"])
+@pytest.mark.parametrize("dealias", ["DealiasXZ", "NoDealiasing"])
+def test_nonlinear_single_call(gpu, method, dealias):
+    Nx, Ny, Nz = 24, 17, 18
+    g = R.Grid(Nx, Ny, Nz, LX_C, LZ_C)
+    u = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.3, decay=0.5, seed=11, dealias=(dealias == "DealiasXZ"))
+    fl = dict(nonlinearity=method, dealiasing=dealias, baseflow="LaminarBase", uupperwall=1.0, ulowerwall=-1.0,
+              constraint="BulkVelocity", nu=1 / 400, Vsuck=0.0)
+    f_ref, _ = R.nonlinear(g, R.make_flags(**fl), u)
+    f_gpu = cfa.nonlinear(cfa.FlowField(Nx, Ny, Nz, 3, LX_C, LZ_C, data=u), cfa.DNSFlags(**fl))
+    kxd = Nx // 3 - 1 if dealias == "DealiasXZ" else Nx // 2 - 1
+    kzd = Nz // 3 - 1 if dealias == "DealiasXZ" else Nz // 2 - 1
+    m = live_mask(Nx, Nz, kxd, kzd)
+    fr = np.where(m[None, None], f_ref, 0)
+    err = rel_l2_per_component(g, f_gpu, fr)
+    print(method, dealias, "rel L2 per component", err, flush=True)
+    assert np.all(err <= 1e-12), err
+
+
+@pytest.mark.parametrize("constraint", ["PressureGradient", "BulkVelocity"])
+@pytest.mark.parametrize("taucorr", [True, False])
+def test_solve_single_call(gpu, constraint, taucorr):
+    Nx, Ny, Nz = 16, 33, 16
+    g = R.Grid(Nx, Ny, Nz, LX_C, LZ_C)
+    rng = np.random.default_rng(5)
+    rhs = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=1.0, decay=0.7, seed=3) + \
+        1e-3 * (rng.standard_normal((3, Ny, Nx, Nz // 2 + 1)) + 1j * rng.standard_normal((3, Ny, Nx, Nz // 2 + 1)))
+    u0 = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.1, decay=0.5, seed=4)
+    fl = dict(constraint=constraint, taucorrection=taucorr, nu=1 / 400, dPdx=-0.01, Ubulk=0.6,
+              baseflow="LaminarBase")
+    lam = (11.0 / 6.0) / 0.02
+    u_ref, q_ref, dp_ref = R.solve(g, R.make_flags(**fl), lam, u0, rhs)
+    u_gpu, q_gpu, dp_gpu = cfa.solve(cfa.FlowField(Nx, Ny, Nz, 3, LX_C, LZ_C, data=u0), rhs, cfa.DNSFlags(**fl), lam)
+    eu = rel_l2_per_component(g, u_gpu, u_ref)
+    eq = rel_l2_per_component(g, q_gpu, q_ref)
+    print(constraint, taucorr, "u", eu, "q", eq, "dPdx", dp_gpu, dp_ref, flush=True)
+    assert np.all(eu <= 1e-12) and np.all(eq <= 1e-12)
+    if constraint == "BulkVelocity":
+        assert abs(dp_gpu - dp_ref) <= 1e-12 * max(1.0, abs(dp_ref))
+
+
+C1_FLAGS = dict(baseflow="LaminarBase", ulowerwall=-1.0, uupperwall=1.0, constraint="BulkVelocity", Ubulk=0.0,
+                timestepping="SBDF3", initstepping="SMRK2", nonlinearity="SkewSymmetric", dealiasing="DealiasXZ",
+                taucorrection=True, nu=1 / 400, dt=0.02)
+
+
+def run_pair(Nx, Ny, Nz, Lx, Lz, flags, u0, steps, chunks=1):
+    g = R.Grid(Nx, Ny, Nz, Lx, Lz)
+    ref = R.OracleDNS(g, R.make_flags(**flags), u0, g.zeros(1))
+    fu = cfa.FlowField(Nx, Ny, Nz, 3, Lx, Lz, data=u0)
+    fq = cfa.FlowField(Nx, Ny, Nz, 1, Lx, Lz)
+    dns = cfa.DNS([fu, fq], cfa.DNSFlags(**flags))
+    for _ in range(chunks):
+        ref.advance(steps // chunks)
+        dns.advance_resident(steps // chunks)
+    ur, qr = ref.state()
+    ug, qg = dns.state()
+    return g, ur, qr, ug, qg, ref, dns
+
+
+def test_dns_c1_20_steps(gpu):
+    """C1 physics (Couette Re=400, skew-symmetric, bulk velocity, SBDF3 + SMRK2) at 32x33x32, 20 steps."""
+    u0 = random_field(32, 33, 32, LX_C, LZ_C, norm=0.1, decay=0.3, seed=1)
+    g, ur, qr, ug, qg, ref, dns = run_pair(32, 33, 32, LX_C, LZ_C, C1_FLAGS, u0, 20, chunks=2)
+    eu = rel_l2_per_component(g, ug, ur)
+    print("C1 20 steps rel L2 per component", eu, "t", dns.time(), ref.time, flush=True)
+    assert np.all(eu <= 1e-10), eu
+    assert abs(dns.time() - ref.time) < 1e-12
+    assert abs(dns.dPdx() - ref.dPdx) <= 1e-10 * max(1.0, abs(ref.dPdx))
+
+
+@pytest.mark.parametrize("method,stepping,init", [("Rotational", "SBDF2", "SMRK2"), ("Convection", "SBDF4", "SMRK2"),
+                                                  ("Divergence", "SBDF1", "SMRK2"), ("Alternating", "CNAB2", "SMRK2"),
+                                                  ("Rotational", "CNRK2", "CNRK2"), ("SkewSymmetric", "SMRK2", "SMRK2")])
+def test_dns_schemes(gpu, method, stepping, init):
+    flags = dict(C1_FLAGS, nonlinearity=method, timestepping=stepping, initstepping=init)
+    u0 = random_field(24, 25, 24, LX_C, LZ_C, norm=0.1, decay=0.3, seed=2)
+    g, ur, qr, ug, qg, ref, dns = run_pair(24, 25, 24, LX_C, LZ_C, flags, u0, 12, chunks=3)
+    eu = rel_l2_per_component(g, ug, ur)
+    print(method, stepping, init, "rel L2", eu, flush=True)
+    assert np.all(eu <= 1e-10), eu
+
+
+def test_golden_trajectory(gpu):
+    """timeIntegrationTest: uinit -> 440 SBDF3 steps -> ufinal, L2Dist <= 1e-13 (the reference's own bar)."""
+    man = json.load(open(os.path.join(HERE, "golden", "manifest.json")))
+    mi = man["uinit"]
+    g = R.Grid(mi["Nx"], mi["Ny"], mi["Nz"], mi["Lx"], mi["Lz"], mi["a"], mi["b"])
+    shape = tuple(mi["shape"])
+    u0 = R.field_from_io(g, np.fromfile(os.path.join(HERE, "golden", "uinit.f64")).reshape(shape))
+    uf = R.field_from_io(g, np.fromfile(os.path.join(HERE, "golden", "ufinal.f64")).reshape(shape))
+    t = man["timeIntegrationTest"]
+    flags = cfa.DNSFlags(baseflow="SuctionBase", timestepping="SBDF3", initstepping="SMRK2",
+                         nonlinearity="Rotational", dealiasing="DealiasXZ", taucorrection=True,
+                         constraint="PressureGradient", dPdx=0.0, dt=t["dt"], nu=t["nu"], Vsuck=t["Vsuck"])
+    fu = cfa.FlowField(mi["Nx"], mi["Ny"], mi["Nz"], 3, mi["Lx"], mi["Lz"], data=u0)
+    fq = cfa.FlowField(mi["Nx"], mi["Ny"], mi["Nz"], 1, mi["Lx"], mi["Lz"])
+    dns = cfa.DNS([fu, fq], flags)
+    for _ in range(11):  # T = 10 in unit steps of 40 (timeIntegrationTest.cpp:104-137)
+        dns.advance([fu, fq], 40)
+    d = np.sqrt(R.l2dist2(g, fu.data, uf))
+    print("golden L2Dist", d, flush=True)
+    assert d <= t["tolerance_L2Dist"], d
