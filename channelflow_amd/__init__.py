@@ -27,7 +27,9 @@ _ENUMS = dict(baseflow=BASEFLOW, constraint=CONSTRAINT, timestepping=STEPPING, i
 EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_destroy", "cfg_dns_advance",
             "cfg_dns_set_state", "cfg_dns_get_state", "cfg_dns_reset_dt", "cfg_dns_reset_time",
             "cfg_dns_reset_gradp", "cfg_dns_reset_bulkv", "cfg_dns_time", "cfg_dns_dpdx", "cfg_dns_ubulk",
-            "cfg_dns_cfl", "cfg_dns_base", "cfg_dns_device_state", "cfg_nonlinear", "cfg_solve")
+            "cfg_dns_cfl", "cfg_dns_base", "cfg_dns_device_state", "cfg_dns_profile", "cfg_dns_kernel_times",
+            "cfg_kernel_name", "cfg_nonlinear", "cfg_solve")
+NKERNELS = 9
 
 
 class CfgError(RuntimeError):
@@ -78,6 +80,9 @@ def library():
         "cfg_dns_cfl": ([v, P], i),
         "cfg_dns_base": ([v, P, P], i),
         "cfg_dns_device_state": ([v, C.POINTER(P), C.POINTER(P), C.POINTER(C.c_int), C.POINTER(C.c_int)], i),
+        "cfg_dns_profile": ([v, i], i),
+        "cfg_dns_kernel_times": ([v, P, C.POINTER(C.c_longlong)], i),
+        "cfg_kernel_name": ([i], C.c_char_p),
         "cfg_nonlinear": ([G, F, P, P], i),
         "cfg_solve": ([G, F, d, P, P, P, P, P], i),
     }
@@ -268,6 +273,17 @@ class DNS:
 
     def reset_bulkv(self, Ubulk, Wbulk=0.0):
         _check(library().cfg_dns_reset_bulkv(self._h, float(Ubulk), float(Wbulk)))
+
+    def profile(self, enable=True):
+        """Enable (and clear) per-kernel HIP-event timing on this DNS's stream."""
+        _check(library().cfg_dns_profile(self._h, int(bool(enable))))
+
+    def kernel_times(self):
+        """{kernel name: (total ms, launches)} since profile(True)."""
+        tot = np.zeros(NKERNELS)
+        cnt = (C.c_longlong * NKERNELS)()
+        _check(library().cfg_dns_kernel_times(self._h, _ptr(tot), cnt))
+        return {library().cfg_kernel_name(i).decode(): (float(tot[i]), int(cnt[i])) for i in range(NKERNELS)}
 
     def Ubase(self):
         U = np.zeros(self._grid.Ny)

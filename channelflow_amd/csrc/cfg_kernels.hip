@@ -17,6 +17,7 @@
 
 #include "cfg_kernels.hpp"
 #include "cfg_math.hpp"
+#include "cfg_wave.hpp"
 
 namespace cfg {
 
@@ -99,22 +100,23 @@ __device__ double2* fft_lds(double2* a, double2* b, const FFTPlan& P, int nlines
 __device__ TauView make_view(const Geom& g, const TauSet& s, int m, double nu, bool taucorr) {
     const int ikx = m % g.nkx, ikz = m / g.nkx;
     const int kx = kx_of(g, ikx), kz = ikz;
-    const long S = g.M;
     const double pi2 = PI * PI;
     const double kappa2 = 4 * pi2 * ((kx / g.Lx) * (kx / g.Lx) + (kz / g.Lz) * (kz / g.Lz));
     const double c = 4.0 * pi2 * nu;
     const double lambda = s.lambda_t + c * ((kx / g.Lx) * (kx / g.Lx) + (kz / g.Lz) * (kz / g.Lz));
+    double* cm = s.coef + (long)m * TAU_NA * g.Ny;
+    auto A = [&](int k) { return SV{cm + (long)k * g.Ny, 1}; };
     TauView t;
-    t.ph = HelmholtzView{SV{s.pband + m, S}, SV{s.pup + m, S}, SV{s.pinvd + m, S}, kappa2, g.Ny};
-    t.vh = HelmholtzView{SV{s.vband + m, S}, SV{s.vup + m, S}, SV{s.vinvd + m, S}, lambda, g.Ny};
-    t.Pp = SV{s.Pp + m, S};
-    t.Pm = SV{s.Pm + m, S};
-    t.vp = SV{s.vp + m, S};
-    t.vm = SV{s.vm + m, S};
-    t.P0 = SV{s.P0 + m, S};
-    t.v0 = SV{s.v0 + m, S};
-    t.scal = s.scal + m;
-    t.sstride = S;
+    t.ph = HelmholtzView{A(0), A(1), A(2), kappa2, g.Ny};
+    t.vh = HelmholtzView{A(3), A(4), A(5), lambda, g.Ny};
+    t.Pp = A(6);
+    t.Pm = A(7);
+    t.vp = A(8);
+    t.vm = A(9);
+    t.P0 = A(10);
+    t.v0 = A(11);
+    t.scal = s.scal + (long)m * 6;
+    t.sstride = 1;
     t.nu = nu;
     t.a = g.a;
     t.b = g.b;
@@ -145,48 +147,199 @@ __device__ __forceinline__ CV cview(double* base, const Geom& g, int comp, int m
     return CV{SV{p, S2}, SV{p + 1, S2}};
 }
 
-__global__ void __launch_bounds__(64) k_tau_solve(Geom g, TauSet s, double nu, int taucorr, SolveConst c, double* rhs,
-                                                  double* u, double* q, double* scratch) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= g.M) return;
-    const long S = g.M;
-    TauView t = make_view(g, s, m, nu, taucorr != 0);
-    CV Rx = cview(rhs, g, 0, m), Ry = cview(rhs, g, 1, m), Rz = cview(rhs, g, 2, m);
-    CV U = cview(u, g, 0, m), V = cview(u, g, 1, m), W = cview(u, g, 2, m), P = cview(q, g, 0, m);
-    SV s0{scratch + m, S}, s1{scratch + S * g.Ny + m, S};
-    if (!t.zero_mode) {
-        tau_solve(t, U, V, W, P, Rx, Ry, Rz, s0, s1);
-        return;
+// Tau solve with one wavefront per live mode (NSE::solve, nse.cpp:479-575;
+// TauSolver::solve, tausolver.cpp:347-450, with solve_P_and_v :193-251).
+constexpr int TAU_WPB = 8;  // waves (modes) per block; 8 complex doubles = one 128-byte row
+
+template <int K>
+__device__ void tau_P_and_v_wave(const WCtx& c, const double* cm, const double* sc, double kappa2, double lambda,
+                                 double nu, bool taucorr, const WV<K>& r, const WV<K>& Ry, WV<K>& P, WV<K>& v) {
+    const long Ny = c.N;
+    P = whelm<K>(r, cm + 0 * Ny, cm + 1 * Ny, cm + 2 * Ny, kappa2, 0.0, 0.0, c);
+    WV<K> tmp = wdiff<K>(P, c);
+    for (int k = 0; k < K; ++k) {
+        tmp.e[k] -= Ry.e[k];
+        tmp.o[k] -= Ry.o[k];
     }
-    // kx = kz = 0 (nse.cpp:505-547): base-flow dissipation and the mean
-    // pressure gradient or bulk-velocity constraint, then real parts only.
-    const int N = g.Ny;
-    for (int n = 0; n < N; ++n) Rx.re[n] += c.nuUbaseyy[n];
-    for (int n = 0; n < N; ++n) Rz.re[n] += c.nuWbaseyy[n];
-    if (!c.bulk) {
-        Rx.re[0] -= c.dPdxRef;
-        Rz.re[0] -= c.dPdzRef;
-        tau_solve(t, U, V, W, P, Rx, Ry, Rz, s0, s1);
-    } else {
-        SV s2{scratch + 2 * S * g.Ny + m, S}, s3{scratch + 3 * S * g.Ny + m, S};
-        double dPdx, dPdz;
-        tau_solve_bulk(t, U, V, W, P, Rx, Ry, Rz, c.umean, c.wmean, s0, s1, s2, s3, dPdx, dPdz);
-        *c.dPdx_out = dPdx;
-        *c.dPdz_out = dPdz;
+    v = whelm<K>(tmp, cm + 3 * Ny, cm + 4 * Ny, cm + 5 * Ny, lambda, 0.0, 0.0, c);
+    // influence-matrix correction (tausolver.cpp:178-191)
+    double dvb, dva;
+    wends<K>(wdiff<K>(v, c), c, dvb, dva);
+    const double dplus = -sc[0] * dvb - sc[1] * dva;
+    const double dminus = -sc[2] * dvb - sc[3] * dva;
+    {
+        const WV<K> Pp = wload<K>(cm + 6 * Ny, c), Pm = wload<K>(cm + 7 * Ny, c);
+        const WV<K> vp = wload<K>(cm + 8 * Ny, c), vm = wload<K>(cm + 9 * Ny, c);
+        for (int k = 0; k < K; ++k) {
+            P.e[k] += dplus * Pp.e[k] + dminus * Pm.e[k];
+            P.o[k] += dplus * Pp.o[k] + dminus * Pm.o[k];
+            v.e[k] += dplus * vp.e[k] + dminus * vm.e[k];
+            v.o[k] += dplus * vp.o[k] + dminus * vm.o[k];
+        }
     }
-    for (int n = 0; n < N; ++n) {
-        U.im[n] = 0.0;
-        V.im[n] = 0.0;
-        W.im[n] = 0.0;
-        P.im[n] = 0.0;
+    if (!taucorr) return;
+    const int Nb = c.Nb;
+    double sigma1_Nb = lambda * wget<K>(v, Nb, c) - nu * 0.0 - wget<K>(Ry, Nb, c);
+    double sigma1_Nb1 = lambda * wget<K>(v, Nb - 1, c) - nu * 0.0 - wget<K>(Ry, Nb - 1, c);
+    sigma1_Nb1 += (4.0 / c.L) * Nb * wget<K>(P, Nb, c);
+    const double sigmaNb = sigma1_Nb / (1.0 - sc[5]);
+    const double sigmaNb1 = sigma1_Nb1 / (1.0 - sc[4]);
+    const WV<K> P0 = wload<K>(cm + 10 * Ny, c), v0 = wload<K>(cm + 11 * Ny, c);
+    for (int k = 0; k < K; ++k) {
+        P.e[k] += sigmaNb1 * P0.e[k];
+        P.o[k] += sigmaNb * P0.o[k];
+        v.e[k] += sigmaNb * v0.e[k];
+        v.o[k] += sigmaNb1 * v0.o[k];
+    }
+}
+
+// Block of TAU_WPB consecutive modes: the right-hand side rows [c][n][m0..m0+WPB)
+// are read with full-line loads into LDS as [mode][c][n], each wave solves its
+// mode in registers, and u, v, w, q leave through LDS the same way.
+template <int K>
+__global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, TauSet s, double nu, int taucorr,
+                                                                 SolveConst zc, const double2* __restrict__ rhs,
+                                                                 double2* __restrict__ u, double2* __restrict__ q) {
+    extern __shared__ double2 lds[];
+    const int Ny = g.Ny;
+    const int nthr = 64 * TAU_WPB;
+    const int m0 = blockIdx.x * TAU_WPB;
+    const int w = threadIdx.x >> 6;
+    const int m = m0 + w;
+    const bool valid = m < g.M;
+    for (int idx = threadIdx.x; idx < 3 * Ny * TAU_WPB; idx += nthr) {
+        const int ww = idx % TAU_WPB, rest = idx / TAU_WPB, n = rest % Ny, comp = rest / Ny;
+        const int mm = m0 + ww;
+        lds[(ww * 3 + comp) * Ny + n] = (mm < g.M) ? rhs[((long)comp * Ny + n) * g.M + mm] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    WCtx c;
+    c.N = Ny;
+    c.Me = (Ny + 1) / 2;
+    c.Mo = (Ny - 1) / 2;
+    c.Nb = Ny - 1;
+    c.Nc = Ny - 1;
+    c.lane = threadIdx.x & 63;
+    c.L = g.b - g.a;
+    const int ikx = m % g.nkx, ikz = m / g.nkx;
+    const int kx = kx_of(g, ikx), kz = ikz;
+    const bool zero = (kx == 0 && kz == 0);
+    const double pi2 = PI * PI;
+    const double kk = (kx / g.Lx) * (kx / g.Lx) + (kz / g.Lz) * (kz / g.Lz);
+    const double kappa2 = 4 * pi2 * kk;
+    const double lambda = s.lambda_t + (4.0 * pi2 * nu) * kk;
+    const double kx2pi = 2 * PI * kx / g.Lx, kz2pi = 2 * PI * kz / g.Lz;
+    const double* cm = s.coef + (long)(valid ? m : 0) * TAU_NA * Ny;
+    const double* sc = s.scal + (long)(valid ? m : 0) * 6;
+    WV<K> Rxr, Rxi, Ryr, Ryi, Rzr, Rzi;
+    if (valid) {
+        wload_c<K>(lds + (w * 3 + 0) * Ny, c, Rxr, Rxi);
+        wload_c<K>(lds + (w * 3 + 1) * Ny, c, Ryr, Ryi);
+        wload_c<K>(lds + (w * 3 + 2) * Ny, c, Rzr, Rzi);
+        if (zero) {  // kx = kz = 0 (nse.cpp:505-527): nu U'', nu W'' and the mean pressure gradient
+            const WV<K> Uyy = wload<K>(zc.nuUbaseyy, c), Wyy = wload<K>(zc.nuWbaseyy, c);
+            for (int k = 0; k < K; ++k) {
+                Rxr.e[k] += Uyy.e[k];
+                Rxr.o[k] += Uyy.o[k];
+                Rzr.e[k] += Wyy.e[k];
+                Rzr.o[k] += Wyy.o[k];
+            }
+            if (!zc.bulk && c.lane == 0) {
+                Rxr.e[0] -= zc.dPdxRef;
+                Rzr.e[0] -= zc.dPdzRef;
+            }
+        }
+    }
+    __syncthreads();
+    WV<K> Pr, Pi;
+    if (valid) {
+        WV<K> vr, vi;
+        if (zero) {  // solve_P_and_v for kx = kz = 0: P from the pressure Helmholtz, v = 0
+            Pr = whelm<K>(wdiff<K>(Ryr, c), cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
+            Pi = whelm<K>(wdiff<K>(Ryi, c), cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
+            vr = wzero<K>();
+            vi = wzero<K>();
+        } else {
+            WV<K> rr = wdiff<K>(Ryr, c);
+            for (int k = 0; k < K; ++k) {
+                rr.e[k] -= kx2pi * Rxi.e[k] + kz2pi * Rzi.e[k];
+                rr.o[k] -= kx2pi * Rxi.o[k] + kz2pi * Rzi.o[k];
+            }
+            tau_P_and_v_wave<K>(c, cm, sc, kappa2, lambda, nu, taucorr != 0, rr, Ryr, Pr, vr);
+            rr = wdiff<K>(Ryi, c);
+            for (int k = 0; k < K; ++k) {
+                rr.e[k] += kx2pi * Rxr.e[k] + kz2pi * Rzr.e[k];
+                rr.o[k] += kx2pi * Rxr.o[k] + kz2pi * Rzr.o[k];
+            }
+            tau_P_and_v_wave<K>(c, cm, sc, kappa2, lambda, nu, taucorr != 0, rr, Ryi, Pi, vi);
+        }
+        if (zero) {  // real parts only (nse.cpp:555-563)
+            Pi = wzero<K>();
+            vi = wzero<K>();
+        }
+        wstore_c<K>(lds + (w * 3 + 0) * Ny, c, Pr, Pi);
+        wstore_c<K>(lds + (w * 3 + 1) * Ny, c, vr, vi);
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 2 * Ny * TAU_WPB; idx += nthr) {
+        const int ww = idx % TAU_WPB, rest = idx / TAU_WPB, n = rest % Ny, slot = rest / Ny;
+        const int mm = m0 + ww;
+        if (mm < g.M) {
+            const double2 v = lds[(ww * 3 + slot) * Ny + n];
+            if (slot == 0) q[(long)n * g.M + mm] = v;
+            else u[((long)1 * Ny + n) * g.M + mm] = v;
+        }
+    }
+    __syncthreads();
+    if (valid) {
+        const double* vb = cm + 3L * Ny;
+        const double* vu = cm + 4L * Ny;
+        const double* vd = cm + 5L * Ny;
+        for (int comp = 0; comp < 3; comp += 2) {
+            const double k2 = comp == 0 ? kx2pi : kz2pi;
+            const WV<K>& Rr = comp == 0 ? Rxr : Rzr;
+            const WV<K>& Ri = comp == 0 ? Rxi : Rzi;
+            WV<K> fr, fi;
+            for (int k = 0; k < K; ++k) {
+                fr.e[k] = -(k2 * Pi.e[k]) - Rr.e[k];
+                fr.o[k] = -(k2 * Pi.o[k]) - Rr.o[k];
+                fi.e[k] = k2 * Pr.e[k] - Ri.e[k];
+                fi.o[k] = k2 * Pr.o[k] - Ri.o[k];
+            }
+            WV<K> xr, xi;
+            if (zero && zc.bulk) {  // bulk-velocity constraint (tausolver.cpp:404-450)
+                const double mu = whelm_mean<K>(fr, vb, vu, vd, lambda, nu, comp == 0 ? zc.umean : zc.wmean, c, xr);
+                if (c.lane == 0) *(comp == 0 ? zc.dPdx_out : zc.dPdz_out) = mu;
+            } else {
+                xr = whelm<K>(fr, vb, vu, vd, lambda, 0.0, 0.0, c);
+            }
+            xi = zero ? wzero<K>() : whelm<K>(fi, vb, vu, vd, lambda, 0.0, 0.0, c);
+            wstore_c<K>(lds + (w * 3 + (comp == 0 ? 0 : 1)) * Ny, c, xr, xi);
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 2 * Ny * TAU_WPB; idx += nthr) {
+        const int ww = idx % TAU_WPB, rest = idx / TAU_WPB, n = rest % Ny, slot = rest / Ny;
+        const int mm = m0 + ww;
+        if (mm < g.M) u[((long)(slot == 0 ? 0 : 2) * Ny + n) * g.M + mm] = lds[(ww * 3 + slot) * Ny + n];
     }
 }
 
 void launch_tau_solve(const Geom& g, const TauSet& s, double nu, bool taucorr, const SolveConst& c, const double* rhs,
                       double* u, double* q, double* scratch, hipStream_t st) {
-    const int blk = 64;
-    k_tau_solve<<<(g.M + blk - 1) / blk, blk, 0, st>>>(g, s, nu, taucorr ? 1 : 0, c, const_cast<double*>(rhs), u, q,
-                                                       scratch);
+    const int blocks = (g.M + TAU_WPB - 1) / TAU_WPB;
+    const size_t lds = (size_t)3 * g.Ny * TAU_WPB * sizeof(double2);
+    const double2* r2 = reinterpret_cast<const double2*>(rhs);
+    double2* u2 = reinterpret_cast<double2*>(u);
+    double2* q2 = reinterpret_cast<double2*>(q);
+    const int Me = (g.Ny + 1) / 2;
+    const int tc = taucorr ? 1 : 0;
+    if (Me <= 64)
+        k_tau_solve_wave<1><<<blocks, 64 * TAU_WPB, lds, st>>>(g, s, nu, tc, c, r2, u2, q2);
+    else if (Me <= 128)
+        k_tau_solve_wave<2><<<blocks, 64 * TAU_WPB, lds, st>>>(g, s, nu, tc, c, r2, u2, q2);
+    else
+        k_tau_solve_wave<3><<<blocks, 64 * TAU_WPB, lds, st>>>(g, s, nu, tc, c, r2, u2, q2);
 }
 
 // ===========================================================================
@@ -680,7 +833,9 @@ static int zpass_xb(const Geom& g, int nphys) {
 }
 
 void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, double Vsuck,
-                      double rotation, const double* u, double* f, const NLWork& w, hipStream_t st) {
+                      double rotation, const double* u, double* f, const NLWork& w, hipStream_t st, KTimer* kt) {
+    KTimer dummy;
+    KTimer& T = kt ? *kt : dummy;
     const dim3 ygrid((g.M + 63) / 64, (g.Ny + YT - 1) / YT);
     const double2* u2 = reinterpret_cast<const double2*>(u);
     double2* yp = reinterpret_cast<double2*>(w.yp);
@@ -691,6 +846,7 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     const int nchunk = (g.nkz + XLB - 1) / XLB;
     const size_t xlds = 2 * XLB * g.Nx * sizeof(double2);
     int nsrc = 3, nxout = 3;
+    hipEvent_t ev = T.begin();
     switch (method) {
         case NL_ROT:
             k_yinv<NL_ROT><<<ygrid, 64, 0, st>>>(g, ym, u2, yp, Vsuck);
@@ -708,19 +864,25 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
             k_yinv<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, u2, yp, Vsuck);
             nsrc = 3; nxout = 3;
     }
+    T.end(K_YINV, ev);
+    ev = T.begin();
     // d/dx outputs (fields nsrc..) derive from fields 0..2 only
     k_xinv<<<dim3(g.Ny * nchunk, nxout), 256, xlds, st>>>(g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, yp, xp);
+    T.end(K_XINV, ev);
     const int nphys = (method == NL_ROT) ? 6 : 9;
     const int XB = zpass_xb(g, nphys);
     const size_t zlds = (size_t)XB * g.Nz * (2 * sizeof(double2) + nphys * sizeof(double));
     const int zgrid = g.Ny * ((g.Nx + XB - 1) / XB);
     const int want_g = (method != NL_DIV) || (rotation != 0.0);
+    ev = T.begin();
     switch (method) {
         case NL_ROT: k_zpass<NL_ROT><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo); break;
         case NL_CONV: k_zpass<NL_CONV><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo); break;
         case NL_SKEW: k_zpass<NL_SKEW><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo); break;
         default: k_zpass<NL_DIV><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo);
     }
+    T.end(K_ZPASS, ev);
+    ev = T.begin();
     // forward x-FFT of the z-pass outputs that exist
     if (want_g) k_xfwd<<<dim3(g.Ny * nchunk, 3), 256, xlds, st>>>(g, px, zo, fo);
     if (method == NL_DIV || method == NL_SKEW) {
@@ -728,12 +890,15 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
         const long offo = 3L * g.Ny * g.M;
         k_xfwd<<<dim3(g.Ny * nchunk, 6), 256, xlds, st>>>(g, px, zo + off, fo + offo);
     }
+    T.end(K_XFWD, ev);
+    ev = T.begin();
     switch (method) {
         case NL_ROT: k_yfwd<NL_ROT><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, f2); break;
         case NL_CONV: k_yfwd<NL_CONV><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, f2); break;
         case NL_SKEW: k_yfwd<NL_SKEW><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, f2); break;
         default: k_yfwd<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, want_g, fo, f2);
     }
+    T.end(K_YFWD, ev);
 }
 
 }  // namespace cfg

@@ -4,7 +4,76 @@
 
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 namespace cfg {
+
+// Per-kernel timing with HIP events on the launching stream (off by default).
+enum KernelId { K_YINV = 0, K_XINV, K_ZPASS, K_XFWD, K_YFWD, K_COMBINE, K_TAU, K_LINEAR, K_SETUP, K_COUNT };
+
+struct KTimer {
+    bool on = false;
+    hipStream_t st = nullptr;
+    struct Rec {
+        int id;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double total_ms[K_COUNT] = {};
+    long count[K_COUNT] = {};
+
+    hipEvent_t take() {
+        hipEvent_t e;
+        if (!pool.empty()) {
+            e = pool.back();
+            pool.pop_back();
+        } else {
+            (void)hipEventCreate(&e);
+        }
+        return e;
+    }
+    hipEvent_t begin() {
+        if (!on) return nullptr;
+        hipEvent_t a = take();
+        (void)hipEventRecord(a, st);
+        return a;
+    }
+    void end(int id, hipEvent_t a) {
+        if (!on || !a) return;
+        hipEvent_t b = take();
+        (void)hipEventRecord(b, st);
+        recs.push_back(Rec{id, a, b});
+        if (recs.size() > 4096) flush();
+    }
+    void flush() {  // waits for the recorded work and accumulates elapsed times
+        if (recs.empty()) return;
+        (void)hipEventSynchronize(recs.back().b);
+        for (auto& r : recs) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, r.a, r.b);
+            total_ms[r.id] += ms;
+            count[r.id] += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        recs.clear();
+    }
+    void reset() {
+        flush();
+        for (int i = 0; i < K_COUNT; ++i) {
+            total_ms[i] = 0.0;
+            count[i] = 0;
+        }
+    }
+    ~KTimer() {
+        for (auto& r : recs) {
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
 
 // Compact spectral layout of one field with nd components:
 //   complex element (c, ny, ikz, ikx) at index ((c*Ny + ny)*nkz + ikz)*nkx + ikx,
@@ -33,9 +102,13 @@ struct YMats {            // y-direction operators, row-major [row][col], Ny x N
     const double* base;   // [4][Ny]: U(y), dU/dy(y), W(y), dW/dy(y) at the y_n
 };
 
-struct TauSet {  // per-mode tau data for one lambda_t, each array [n][M] (scal: [6][M])
-    double *pband, *pup, *pinvd, *vband, *vup, *vinvd;
-    double *Pp, *Pm, *vp, *vm, *P0, *v0;
+// Per-mode tau data for one lambda_t, mode-major: array k (TAU_NA of them:
+// pband pup pinvd vband vup vinvd Pp Pm vp vm P0 v0) of mode m is the Ny-vector
+// at coef + (m*TAU_NA + k)*Ny; scalars i00 i01 i10 i11 sigma0_Nb1 sigma0_Nb at
+// scal + m*6.
+constexpr int TAU_NA = 12;
+struct TauSet {
+    double* coef;
     double* scal;
     double lambda_t;
 };
@@ -78,6 +151,7 @@ struct NLWork {
     double* fo;  // [9][Ny][M] complex
 };
 void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method,
-                      double Vsuck, double rotation, const double* u, double* f, const NLWork& w, hipStream_t st);
+                      double Vsuck, double rotation, const double* u, double* f, const NLWork& w, hipStream_t st,
+                      KTimer* kt = nullptr);
 
 }  // namespace cfg

@@ -34,25 +34,47 @@ struct SV {  // strided view of one Chebyshev profile
 // Chebyshev primitives on [a,b] (channelflow/chebyshev.cpp:405-512, 672-697).
 // ---------------------------------------------------------------------------
 
-// du = d/dy u by the backward three-term recurrence, du[0] halved.
+// du = d/dy u by the backward three-term recurrence, du[0] halved. The two
+// newest terms are carried in registers, so no step waits on a store.
 CFG_HD void cheb_diff(SV u, SV du, int N, double L) {
     const int Nb = N - 1;
     const double scale = 4.0 / L;
-    du[Nb] = 0.0;
-    du[Nb - 1] = scale * Nb * u[Nb];
-    for (int n = Nb - 2; n >= 0; --n) du[n] = du[n + 2] + scale * (n + 1) * u[n + 1];
-    du[0] *= 0.5;
+    double d2 = 0.0, d1 = scale * Nb * u[Nb];
+    du[Nb] = d2;
+    du[Nb - 1] = d1;
+#pragma unroll 4
+    for (int n = Nb - 2; n > 0; --n) {
+        const double dn = d2 + scale * (n + 1) * u[n + 1];
+        du[n] = dn;
+        d2 = d1;
+        d1 = dn;
+    }
+    du[0] = (d2 + scale * 1 * u[1]) * 0.5;
 }
 
 // Values of d/dy u at y=b and y=a, summed from the highest mode down exactly
-// as eval_b()/eval_a() sum the stored derivative; du is scratch.
+// as eval_b()/eval_a() sum the stored derivative. The derivative itself is not
+// stored; du is unused scratch kept for the call signature.
 CFG_HD void cheb_diff_ends(SV u, SV du, int N, double L, double& at_b, double& at_a) {
-    cheb_diff(u, du, N, L);
+    const int Nb = N - 1;
+    const double scale = 4.0 / L;
+    double d2 = 0.0, d1 = scale * Nb * u[Nb];
     double sb = 0.0, sa = 0.0;
-    for (int n = N - 1; n >= 0; --n) {
-        sb += du[n];
-        sa += du[n] * ((n % 2 == 0) ? 1 : -1);
+    sb += d2;
+    sa += d2 * ((Nb % 2 == 0) ? 1 : -1);
+    sb += d1;
+    sa += d1 * (((Nb - 1) % 2 == 0) ? 1 : -1);
+#pragma unroll 4
+    for (int n = Nb - 2; n > 0; --n) {
+        const double dn = d2 + scale * (n + 1) * u[n + 1];
+        sb += dn;
+        sa += dn * ((n % 2 == 0) ? 1 : -1);
+        d2 = d1;
+        d1 = dn;
     }
+    const double d0 = (d2 + scale * 1 * u[1]) * 0.5;
+    sb += d0;
+    sa += d0;
     at_b = sb;
     at_a = sa;
 }
@@ -162,12 +184,26 @@ CFG_HD void helm_solve(const HelmholtzView& h, SV u, SV f, double ua, double ub)
     u[1] = (ub - ua) / 2;
     for (int par = 0; par < 2; ++par) {
         const int M = (par == 0) ? Nc / 2 + 1 : Nc / 2;
-        int i;
-        for (i = M - 2; i > 0; --i) u[2 * i + par] -= h.up[2 * i + par] * u[2 * (i + 1) + par];
-        for (int j = 1; j < M; ++j) u[par] -= h.band[2 * j + par] * u[2 * j + par];
-        u[par] /= h.band[par];
-        for (i = 1; i < M; ++i)
-            u[2 * i + par] = (u[2 * i + par] - helm_lo(2 * i + par, lambda, Nc) * u[2 * (i - 1) + par]) * h.invd[2 * i + par];
+        // U y = b, backward (bandedtridiag.cpp:238-243), carrying b[i+1]
+        double nxt = u[2 * (M - 1) + par];
+#pragma unroll 4
+        for (int i = M - 2; i > 0; --i) {
+            const double v = u[2 * i + par] - h.up[2 * i + par] * nxt;
+            u[2 * i + par] = v;
+            nxt = v;
+        }
+        double acc = u[par];
+#pragma unroll 4
+        for (int j = 1; j < M; ++j) acc -= h.band[2 * j + par] * u[2 * j + par];
+        double prev = acc / h.band[par];
+        u[par] = prev;
+        // L x = y, forward, carrying x[i-1]
+#pragma unroll 4
+        for (int i = 1; i < M; ++i) {
+            const double v = (u[2 * i + par] - helm_lo(2 * i + par, lambda, Nc) * prev) * h.invd[2 * i + par];
+            u[2 * i + par] = v;
+            prev = v;
+        }
     }
 }
 

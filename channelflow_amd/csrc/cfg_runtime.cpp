@@ -215,10 +215,6 @@ struct Geometry {
     DBuf T, TD, Ti, DTi;
 
     void build(const cfg_grid& gr, bool dealias) {
-        if (gr.Nx < 4 || gr.Nz < 4 || gr.Ny < 5 || gr.Ny % 2 == 0)
-            throw Error(CFG_ERR_ARG, "grid must have Nx, Nz >= 4 and odd Ny >= 5");
-        if (gr.Nx % 2 || gr.Nz % 2) throw Error(CFG_ERR_ARG, "Nx and Nz must be even");
-        if (!(gr.b > gr.a) || !(gr.Lx > 0) || !(gr.Lz > 0)) throw Error(CFG_ERR_ARG, "invalid box");
         g.Nx = gr.Nx;
         g.Ny = gr.Ny;
         g.Nz = gr.Nz;
@@ -318,6 +314,22 @@ static void unpack(const Geom& g, int nd, const std::vector<double>& in, double*
                 }
 }
 
+static int smooth235(int n) {
+    for (int p : {2, 3})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+static void check_grid(const cfg_grid& gr) {
+    if (gr.Nx < 4 || gr.Nz < 4 || gr.Ny < 5 || gr.Ny % 2 == 0)
+        throw Error(CFG_ERR_ARG, "grid must have Nx, Nz >= 4 and odd Ny >= 5 (Chebyshev-tau needs odd Ny)");
+    if (gr.Nx % 2 || gr.Nz % 2) throw Error(CFG_ERR_ARG, "Nx and Nz must be even");
+    if (!smooth235(gr.Nx) || !smooth235(gr.Nz)) throw Error(CFG_ERR_ARG, "Nx and Nz must be of the form 2^a 3^b");
+    if (gr.Ny > 383) throw Error(CFG_ERR_ARG, "Ny > 383 is not supported by the tau-solve kernels");
+    if (gr.Nx > 256 || gr.Nz > 512) throw Error(CFG_ERR_ARG, "Nx <= 256 and Nz <= 512 are supported");
+    if (!(gr.b > gr.a) || !(gr.Lx > 0) || !(gr.Lz > 0)) throw Error(CFG_ERR_ARG, "invalid box");
+}
+
 static void check_flags(const cfg_flags& f) {
     if (f.dealiasing == CFG_DealiasY || f.dealiasing == CFG_DealiasXYZ)
         throw Error(CFG_ERR_ARG, "DealiasY / DealiasXYZ (Chebyshev-direction dealiasing) is not supported");
@@ -339,9 +351,11 @@ struct Context {
     DBuf nlw_yp, nlw_xp, nlw_zo, nlw_fo, scratch;
     long nfield = 0;  // doubles in one 3-component compact field
     long nscal = 0;   // doubles in one 1-component compact field
+    KTimer kt;
 
     Context(const cfg_grid& gr, bool dealias) {
         HIPCHK(hipStreamCreate(&st));
+        kt.st = st;
         geo.build(gr, dealias);
         const Geom& g = geo.g;
         nfield = 2L * 3 * g.Ny * g.M;
@@ -384,7 +398,7 @@ static void swap_fields(Fields& a, Fields& b) {
 // NSE (nse.cpp): base flow, constraint bookkeeping, nonlinear/linear/solve
 // ---------------------------------------------------------------------------
 struct TauStore {
-    std::vector<DBuf> arrays;  // 12 profile arrays + scal
+    DBuf coef, scal;
     TauSet set{};
 };
 
@@ -438,15 +452,14 @@ class NSE {
         tau_.resize(lambda_t.size());
         for (size_t j = 0; j < lambda_t.size(); ++j) {
             TauStore& ts = tau_[j];
-            ts.arrays.resize(13);
-            for (int k = 0; k < 12; ++k) ts.arrays[k].alloc(prof);
-            ts.arrays[12].alloc(6 * (size_t)g.M);
-            double** dst[12] = {&ts.set.pband, &ts.set.pup, &ts.set.pinvd, &ts.set.vband, &ts.set.vup, &ts.set.vinvd,
-                                &ts.set.Pp,    &ts.set.Pm,  &ts.set.vp,    &ts.set.vm,    &ts.set.P0,  &ts.set.v0};
-            for (int k = 0; k < 12; ++k) *dst[k] = ts.arrays[k].p;
-            ts.set.scal = ts.arrays[12].p;
+            ts.coef.alloc((size_t)TAU_NA * prof);
+            ts.scal.alloc(6 * (size_t)g.M);
+            ts.set.coef = ts.coef.p;
+            ts.set.scal = ts.scal.p;
             ts.set.lambda_t = lambda_t[j];
+            hipEvent_t ev = c_.kt.begin();
             launch_tau_setup(g, ts.set, flags_.nu, flags_.taucorrection != 0, c_.scratch.p, c_.st);
+            c_.kt.end(K_SETUP, ev);
         }
         HIPCHK(hipGetLastError());
     }
@@ -464,7 +477,7 @@ class NSE {
         }
         YMats ym{c_.geo.T.p, c_.geo.TD.p, c_.geo.Ti.p, c_.geo.DTi.p, base_.p};
         launch_nonlinear(c_.geo.g, ym, c_.geo.fx.plan, c_.geo.fz.plan, m, flags_.Vsuck, flags_.rotation, u, f,
-                         c_.work(), c_.st);
+                         c_.work(), c_.st, &c_.kt);
         HIPCHK(hipGetLastError());
     }
 
@@ -472,7 +485,9 @@ class NSE {
     void linear(const double* u, const double* q, double* out) {
         LinearConst lc{nuUyy_.p, nuWyy_.p, dPdxRef_, dPdzRef_, flags_.constraint == CFG_BulkVelocity ? 1 : 0,
                        nuUslope_, nuWslope_};
+        hipEvent_t ev = c_.kt.begin();
         launch_linear(c_.geo.g, flags_.nu, lc, u, q, out, c_.scratch.p, c_.st);
+        c_.kt.end(K_LINEAR, ev);
         HIPCHK(hipGetLastError());
     }
 
@@ -480,8 +495,10 @@ class NSE {
     void solve(double* u, double* q, double* rhs, int s) {
         SolveConst sc{nuUyy_.p, nuWyy_.p, dPdxRef_, dPdzRef_, UbulkRef_ - UbulkBase_, WbulkRef_ - WbulkBase_,
                       flags_.constraint == CFG_BulkVelocity ? 1 : 0, dP_.p, dP_.p + 1};
+        hipEvent_t ev = c_.kt.begin();
         launch_tau_solve(c_.geo.g, tau_.at(s).set, flags_.nu, flags_.taucorrection != 0, sc, rhs, u, q,
                          c_.scratch.p, c_.st);
+        c_.kt.end(K_TAU, ev);
         HIPCHK(hipGetLastError());
         if (flags_.constraint == CFG_BulkVelocity) dP_dirty_ = true;
     }
@@ -605,7 +622,9 @@ class Algorithm {
             in[n] = t.second;
             ++n;
         }
+        hipEvent_t ev = c_.kt.begin();
         launch_combine(c_.nfield, n, coef, in, out, c_.st);
+        c_.kt.end(K_COMBINE, ev);
     }
     Context& c_;
     std::shared_ptr<NSE> nse_;
@@ -668,7 +687,9 @@ class Multistep : public Algorithm {  // dnsalgo.cpp:102-315
                 coef[n] = -beta_[j];
                 in[n++] = nonlf_[j].p;
             }
+            hipEvent_t ev = c_.kt.begin();
             launch_combine(c_.nfield, n, coef, in, rhs_.p, c_.st);
+            c_.kt.end(K_COMBINE, ev);
             nse_->solve(fields_[J].u.p, fields_[J].q.p, rhs_.p, 0);
             rotate();
             t_ += flags_.dt;
@@ -871,6 +892,7 @@ int cfg_dns_create(const cfg_grid* grid, const cfg_flags* flags, const double* u
     return guarded([&] {
         if (!grid || !flags || !u_spec || !out) throw Error(CFG_ERR_ARG, "null argument");
         check_flags(*flags);
+        check_grid(*grid);
         auto d = std::make_unique<cfg_dns>();
         d->flags = *flags;
         d->grid = *grid;
@@ -1015,10 +1037,36 @@ int cfg_dns_device_state(cfg_dns* dns, double** u_dev, double** q_dev, int* nkx,
     });
 }
 
+int cfg_dns_profile(cfg_dns* dns, int enable) {
+    return guarded([&] {
+        if (!dns) throw Error(CFG_ERR_ARG, "null dns");
+        dns->ctx->kt.reset();
+        dns->ctx->kt.on = enable != 0;
+    });
+}
+
+int cfg_dns_kernel_times(cfg_dns* dns, double* total_ms, long long* counts) {
+    return guarded([&] {
+        if (!dns) throw Error(CFG_ERR_ARG, "null dns");
+        dns->ctx->kt.flush();
+        for (int i = 0; i < K_COUNT; ++i) {
+            if (total_ms) total_ms[i] = dns->ctx->kt.total_ms[i];
+            if (counts) counts[i] = dns->ctx->kt.count[i];
+        }
+    });
+}
+
+const char* cfg_kernel_name(int id) {
+    static const char* names[K_COUNT] = {"k_yinv", "k_xinv", "k_zpass", "k_xfwd", "k_yfwd",
+                                         "k_combine", "k_tau_solve", "k_linear", "k_tau_setup"};
+    return (id >= 0 && id < K_COUNT) ? names[id] : "";
+}
+
 int cfg_nonlinear(const cfg_grid* grid, const cfg_flags* flags, const double* u_spec, double* f_spec) {
     return guarded([&] {
         if (!grid || !flags || !u_spec || !f_spec) throw Error(CFG_ERR_ARG, "null argument");
         check_flags(*flags);
+        check_grid(*grid);
         Context c(*grid, flags->dealiasing == CFG_DealiasXZ);
         Fields u;
         u.alloc(c);
@@ -1036,6 +1084,7 @@ int cfg_solve(const cfg_grid* grid, const cfg_flags* flags, double lambda_t, con
     return guarded([&] {
         if (!grid || !flags || !u0_spec || !rhs_spec || !u_spec || !q_spec) throw Error(CFG_ERR_ARG, "null argument");
         check_flags(*flags);
+        check_grid(*grid);
         Context c(*grid, flags->dealiasing == CFG_DealiasXZ);
         const Geom& g = c.geo.g;
         Fields u;

@@ -114,41 +114,36 @@ def random_field(Nx, Ny, Nz, Lx, Lz, norm=0.1, decay=0.3, seed=0, member=0, deal
     kzd = Nz // 3 - 1 if dealias else Nz // 2 - 1
     nv = Ny - 4      # coefficients of p: (1-y^2)^2 p has degree < Ny
     ne = Ny - 2      # coefficients of q and r
-    L = 2.0  # interval [-1, 1]
-    for kx in range(-kxd, kxd + 1):
-        for kz in range(0, kzd + 1):
-            if kz == 0 and kx < 0:
-                continue
-            mx = kx % Nx
-            if kx == 0 and kz == 0:
-                if not meanflow:
-                    continue
-                n = np.arange(ne)
-                r0 = _gauss(seed, member, 3, 0, 0, n).real * decay ** (n / 4.0)
-                r2 = _gauss(seed, member, 4, 0, 0, n).real * decay ** (n / 4.0)
-                u[0, :, 0, 0] = one_minus_y2(r0)[:Ny]
-                u[2, :, 0, 0] = one_minus_y2(r2)[:Ny]
-                continue
-            scale_k = decay ** (abs(kx) + abs(kz))
-            n = np.arange(nv)
-            p = _gauss(seed, member, 0, kx, kz, n) * scale_k * decay ** (n / 4.0)
-            n2 = np.arange(ne)
-            q = _gauss(seed, member, 1, kx, kz, n2) * scale_k * decay ** (n2 / 4.0)
-            v = one_minus_y2(one_minus_y2(p))[:Ny]
-            eta = one_minus_y2(q)[:Ny]
-            dv = cheb_diff(v, L)
-            alpha = 2 * np.pi * kx / Lx
-            gamma = 2 * np.pi * kz / Lz
-            k2 = alpha * alpha + gamma * gamma
-            uu = 1j * (alpha * dv - gamma * eta) / k2
-            ww = 1j * (gamma * dv + alpha * eta) / k2
-            u[0, :, mx, kz] = uu
-            u[1, :, mx, kz] = v
-            u[2, :, mx, kz] = ww
-            if kz == 0:
-                u[0, :, (-kx) % Nx, 0] = np.conj(uu)
-                u[1, :, (-kx) % Nx, 0] = np.conj(v)
-                u[2, :, (-kx) % Nx, 0] = np.conj(ww)
+    KX, KZ = np.meshgrid(np.arange(-kxd, kxd + 1), np.arange(0, kzd + 1), indexing="ij")
+    keep = ~((KZ == 0) & (KX <= 0))             # kz = 0, kx < 0 follow by conjugation; (0,0) below
+    kx = KX[keep][:, None]
+    kz = KZ[keep][:, None]
+    scale_k = decay ** (np.abs(kx) + np.abs(kz))
+    n = np.arange(nv)[None, :]
+    p = _gauss(seed, member, 0, kx, kz, n) * scale_k * decay ** (n / 4.0)
+    n2 = np.arange(ne)[None, :]
+    q = _gauss(seed, member, 1, kx, kz, n2) * scale_k * decay ** (n2 / 4.0)
+    v = one_minus_y2(one_minus_y2(p))[:, :Ny]
+    eta = one_minus_y2(q)[:, :Ny]
+    dv = cheb_diff(v, 2.0)
+    alpha = 2 * np.pi * kx / Lx
+    gamma = 2 * np.pi * kz / Lz
+    k2 = alpha * alpha + gamma * gamma
+    uu = 1j * (alpha * dv - gamma * eta) / k2
+    ww = 1j * (gamma * dv + alpha * eta) / k2
+    mx = (kx[:, 0] % Nx)
+    mz = kz[:, 0]
+    for comp, val in ((0, uu), (1, v), (2, ww)):
+        u[comp][:, mx, mz] = val.T
+    z0 = mz == 0                                # Hermitian kz = 0 plane
+    mxc = (-kx[z0, 0]) % Nx
+    for comp, val in ((0, uu), (1, v), (2, ww)):
+        u[comp][:, mxc, 0] = np.conj(val[z0]).T
+    if meanflow:
+        r0 = _gauss(seed, member, 3, 0, 0, np.arange(ne)).real * decay ** (np.arange(ne) / 4.0)
+        r2 = _gauss(seed, member, 4, 0, 0, np.arange(ne)).real * decay ** (np.arange(ne) / 4.0)
+        u[0, :, 0, 0] = one_minus_y2(r0)[:Ny]
+        u[2, :, 0, 0] = one_minus_y2(r2)[:Ny]
     nrm = np.sqrt(l2norm2(u, live=(kxd, kzd)))
     if nrm > 0 and norm is not None:
         u *= norm / nrm

@@ -91,6 +91,15 @@ int cfg_dns_base(const cfg_dns* dns, double* Ubase, double* Wbase);
  * ikx - nkx otherwise, kz = ikz. */
 int cfg_dns_device_state(cfg_dns* dns, double** u_dev, double** q_dev, int* nkx, int* nkz);
 
+/* Per-kernel timing with HIP events on the DNS's stream (no counterpart in
+ * the reference; used by the benchmark). cfg_dns_profile(dns, 1) clears and
+ * enables the counters; cfg_dns_kernel_times fills total milliseconds and
+ * launch counts for the CFG_NKERNELS kernel classes named by cfg_kernel_name. */
+#define CFG_NKERNELS 9
+int cfg_dns_profile(cfg_dns* dns, int enable);
+int cfg_dns_kernel_times(cfg_dns* dns, double* total_ms, long long* counts);
+const char* cfg_kernel_name(int id);
+
 /* One NSE::nonlinear evaluation (nse.cpp:383-391): base flow added,
  * nonlinearity per flags.nonlinearity, aliased modes of f zero. */
 int cfg_nonlinear(const cfg_grid* grid, const cfg_flags* flags, const double* u_spec, double* f_spec);

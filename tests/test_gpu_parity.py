@@ -142,3 +142,35 @@ def test_golden_trajectory(gpu):
     d = np.sqrt(R.l2dist2(g, fu.data, uf))
     print("golden L2Dist", d, flush=True)
     assert d <= t["tolerance_L2Dist"], d
+
+
+def test_c2_full_size_20_steps(gpu):
+    """192x129x192 Poiseuille (bench workload), 20 steps, against the oracle's committed
+    fixture (tests/golden/make_c2_reference.py): per-component norms and every
+    coefficient of 12 sampled modes at 1e-10; divergence-free and no-slip at full size."""
+    import sys
+    sys.path.insert(0, os.path.dirname(HERE))
+    from bench import workload
+    from channelflow_amd.fields import divergence
+    meta = json.load(open(os.path.join(HERE, "golden", "c2_20steps.json")))
+    ref = np.fromfile(os.path.join(HERE, "golden", "c2_20steps_modes.c128"), dtype=np.complex128)
+    ref = ref.reshape(meta["shape"])
+    wl = workload("c2")
+    Nx, Ny, Nz = wl["grid"]
+    u0 = random_field(Nx, Ny, Nz, wl["Lx"], wl["Lz"], norm=meta["norm"], decay=meta["decay"], seed=meta["seed"])
+    fu = cfa.FlowField(Nx, Ny, Nz, 3, wl["Lx"], wl["Lz"], data=u0)
+    fq = cfa.FlowField(Nx, Ny, Nz, 1, wl["Lx"], wl["Lz"])
+    dns = cfa.DNS([fu, fq], cfa.DNSFlags(**wl["flags"]))
+    dns.advance_resident(meta["steps"])
+    u, q = dns.state()
+    g = R.Grid(Nx, Ny, Nz, wl["Lx"], wl["Lz"])
+    norms = np.array([np.sqrt(R.l2norm2(g, u[i:i + 1])) for i in range(3)])
+    en = np.abs(norms - np.array(meta["l2norm_u"])) / np.array(meta["l2norm_u"])
+    got = np.stack([u[:, :, kx % Nx, kz] for kx, kz in meta["modes"]], axis=0)
+    em = np.array([np.linalg.norm(got[:, i] - ref[:, i]) / np.linalg.norm(ref[:, i]) for i in range(3)])
+    div = np.abs(divergence(u, wl["Lx"], wl["Lz"])).max() / np.abs(u).max()
+    walls = max(np.abs(u.sum(axis=1)).max(), np.abs((u * (-1.0) ** np.arange(Ny)[None, :, None, None]).sum(axis=1)).max())
+    print("C2 20 steps: norm rel diff", en, "sampled-mode rel L2", em, "div", div, "wall", walls, flush=True)
+    assert np.all(en <= 1e-10) and np.all(em <= 1e-10)
+    assert div < 1e-11 and walls < 1e-12
+    assert abs(dns.time() - meta["time"]) < 1e-12

@@ -1,0 +1,232 @@
+"""The oracle (oracle/cpu_ref.cpp) pinned against the reference's own fixtures
+and closed forms (CPU only).
+
+Pins, by reference test:
+  * timeIntegrationTest (tests/timeIntegrationTest.cpp:42-137): uinit -> 440
+    SBDF3 steps -> ufinal at L2Dist <= 1e-13 -- rotational NL, SMRK2 start,
+    tau solver, ASBL base flow, transforms, end to end.
+  * chebyTest (tests/chebyTest.cpp:17-93): transform round trip and d/dy on
+    known functions.
+  * helmholtzTest (tests/helmholtzTest.cpp:36-152): manufactured Dirichlet and
+    mean-constraint problems.
+  * tridiagTest (tests/tridiagTest.cpp:16-95): banded UL solve against a dense
+    solve.
+  * tausolverTest (tests/tausolverTest.cpp:47-146): an exact divergence-free
+    (u, v, w, P) satisfying the no-slip BCs -> R -> solve recovers it.
+  * dnsParabolaTest (tests/dnsParabolaTest.cpp:60-92, 154-248): the Poiseuille
+    parabola is a steady state.
+  * skew = (convection + divergence)/2 and rotational = convection - grad(|u|^2/2)
+    identities for the nonlinear term.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cfref as R
+from channelflow_amd.fields import random_field, divergence, cheb_diff
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LX, LZ = 2 * np.pi / 1.14, 2 * np.pi / 2.5
+
+
+def cheb_values(c, y):
+    return np.polynomial.chebyshev.chebval(y, c)
+
+
+def test_golden_trajectory_oracle():
+    man = json.load(open(os.path.join(HERE, "golden", "manifest.json")))
+    mi = man["uinit"]
+    g = R.Grid(mi["Nx"], mi["Ny"], mi["Nz"], mi["Lx"], mi["Lz"], mi["a"], mi["b"])
+    shape = tuple(mi["shape"])
+    u0 = R.field_from_io(g, np.fromfile(os.path.join(HERE, "golden", "uinit.f64")).reshape(shape))
+    uf = R.field_from_io(g, np.fromfile(os.path.join(HERE, "golden", "ufinal.f64")).reshape(shape))
+    t = man["timeIntegrationTest"]
+    fl = R.make_flags(baseflow="SuctionBase", timestepping="SBDF3", initstepping="SMRK2", nonlinearity="Rotational",
+                      dealiasing="DealiasXZ", taucorrection=True, constraint="PressureGradient", dPdx=0.0,
+                      dt=t["dt"], nu=t["nu"], Vsuck=t["Vsuck"])
+    d = R.OracleDNS(g, fl, u0, g.zeros(1))
+    for _ in range(11):
+        d.advance(40)
+    u, _ = d.state()
+    dist = np.sqrt(R.l2dist2(g, u, uf))
+    assert dist <= t["tolerance_L2Dist"], dist
+    assert abs(d.time - 11.0) < 1e-9
+
+
+def test_cheby_roundtrip_and_diff():
+    # chebyTest: u = 1 + sin(y) sampled at the Chebyshev points; transform,
+    # differentiate, compare with cos(y) (N = 17 on [-1, 1] to spectral accuracy).
+    N = 33
+    y = np.cos(np.pi * np.arange(N) / (N - 1))
+    x = 1 + np.sin(y)
+    c = x.copy()
+    R.lib().cpuref_cheb_spectral(N, R.ptr(c))
+    back = c.copy()
+    R.lib().cpuref_cheb_physical(N, R.ptr(back))
+    assert np.max(np.abs(back - x)) < 1e-14
+    assert np.max(np.abs(cheb_values(c, y) - x)) < 1e-14
+    dc = np.zeros(N)
+    R.lib().cpuref_cheb_diff(N, -1.0, 1.0, R.ptr(c), R.ptr(dc))
+    assert np.max(np.abs(cheb_values(dc, y) - np.cos(y))) < 1e-13
+    # exact on a monomial, interval [a, b] = [-1.1, 1.4] (chebyTest bounds)
+    a, b = -1.1, 1.4
+    coef = np.zeros(N)
+    coef[5] = 1.0
+    dcoef = np.zeros(N)
+    R.lib().cpuref_cheb_diff(N, a, b, R.ptr(coef), R.ptr(dcoef))
+    ref = np.polynomial.chebyshev.chebder(coef) * 2 / (b - a)
+    assert np.max(np.abs(dcoef[:len(ref)] - ref)) < 1e-13
+    # mean: integral over [-1, 1] / 2 of T_n
+    for n in range(0, 9):
+        cc = np.zeros(N)
+        cc[n] = 1.0
+        exact = 0.0 if n % 2 else (1.0 if n == 0 else -1.0 / (n * n - 1))
+        assert abs(R.lib().cpuref_cheb_mean(N, R.ptr(cc)) - exact) < 1e-15
+
+
+@pytest.mark.parametrize("lam,nu", [(0.0, 1.0), (3.7, 1.0), (412.0, 0.0025), (1e4, 1 / 400)])
+def test_helmholtz_manufactured(lam, nu):
+    # helmholtzTest: nu u'' - lambda u = f with u known; the tau solution on N
+    # modes converges to the truth.
+    N = 49
+    y = np.cos(np.pi * np.arange(N) / (N - 1))
+    u_true = np.exp(y) * np.sin(2 * y) + 0.3
+    upp = np.exp(y) * (np.sin(2 * y) + 4 * np.cos(2 * y) - 4 * np.sin(2 * y))
+    f = nu * upp - lam * u_true
+    fc = f.copy()
+    R.lib().cpuref_cheb_spectral(N, R.ptr(fc))
+    uc = np.zeros(N)
+    ua, ub = u_true[-1], u_true[0]
+    assert R.lib().cpuref_helmholtz(N, -1.0, 1.0, lam, nu, R.ptr(fc), ua, ub, R.ptr(uc)) == 0
+    err = np.max(np.abs(cheb_values(uc, y) - u_true))
+    assert err < 1e-11, err
+
+
+def test_helmholtz_mean_constraint():
+    # nu u'' - lambda u = f + mu, mean(u) = umean, u(+-1) = 0 (helmholtzTest second problem)
+    N = 49
+    nu, lam, umean = 0.01, 25.0, 0.37
+    y = np.cos(np.pi * np.arange(N) / (N - 1))
+    f = np.cos(3 * y) * (1 - y * y)
+    fc = f.copy()
+    R.lib().cpuref_cheb_spectral(N, R.ptr(fc))
+    uc = np.zeros(N)
+    mu = np.zeros(1)
+    assert R.lib().cpuref_helmholtz_mean(N, -1.0, 1.0, lam, nu, R.ptr(fc), umean, 0.0, 0.0, R.ptr(uc), R.ptr(mu)) == 0
+    assert abs(R.lib().cpuref_cheb_mean(N, R.ptr(uc)) - umean) < 1e-13
+    # residual of the ODE with the returned mu (all but the two tau rows)
+    d1 = np.polynomial.chebyshev.chebder(uc)
+    d2 = np.polynomial.chebyshev.chebder(d1)
+    res = nu * np.pad(d2, (0, 2)) - lam * uc - fc
+    res[0] -= mu[0]
+    assert np.max(np.abs(res[:N - 2])) < 1e-11
+    assert abs(cheb_values(uc, 1.0)) < 1e-13 and abs(cheb_values(uc, -1.0)) < 1e-13
+
+
+def test_banded_tridiag():
+    # tridiagTest: random row-0-banded tridiagonal, UL without pivoting vs dense solve
+    rng = np.random.default_rng(0)
+    M = 100
+    band = rng.uniform(0.5, 1.5, M)
+    lo = rng.uniform(-0.3, 0.3, M)
+    up = rng.uniform(-0.3, 0.3, M)
+    dg = rng.uniform(2.0, 3.0, M)
+    A = np.zeros((M, M))
+    A[0, :] = band
+    for i in range(1, M):
+        A[i, i] = dg[i]
+        A[i, i - 1] = lo[i]
+        if i + 1 < M:
+            A[i, i + 1] = up[i]
+    dg[0] = band[0]
+    b = rng.standard_normal(M)
+    x = b.copy()
+    assert R.lib().cpuref_banded_solve(M, R.ptr(band), R.ptr(lo), R.ptr(dg), R.ptr(up), R.ptr(x)) == 0
+    xr = np.linalg.solve(A, b)
+    assert np.max(np.abs(x - xr)) < 1e-12 * max(1, np.max(np.abs(xr)))
+
+
+def _deriv_y(c, L=2.0):
+    return cheb_diff(c, L)
+
+
+@pytest.mark.parametrize("kx,kz", [(1, 0), (0, 2), (3, 4), (-2, 1)])
+def test_tausolver_exact_solution(kx, kz):
+    # tausolverTest: build a divergence-free no-slip (u, v, w) and a pressure P,
+    # form R = -(nu u'' - lambda u - grad P) ... solve, verify recovery.
+    N = 49
+    nu, lam_t = 1 / 400, 11 / 6 / 0.02
+    a_, g_ = 2 * np.pi * kx / LX, 2 * np.pi * kz / LZ
+    k2 = a_ * a_ + g_ * g_
+    lam = lam_t + nu * k2
+    full = random_field(3 * max(abs(kx), 1) * 3 + 3, N, 3 * max(kz, 1) * 3 + 3, LX, LZ, norm=None, decay=0.5,
+                        seed=17, dealias=False)
+    mx = kx % full.shape[2]
+    u, v, w = full[0, :, mx, kz], full[1, :, mx, kz], full[2, :, mx, kz]
+    P = np.zeros(N, complex)
+    P[:N - 4] = np.exp(-0.3 * np.arange(N - 4)) * (0.7 + 0.2j)
+    d2 = lambda c: _deriv_y(_deriv_y(c))
+    # momentum: nu u'' - lambda u - grad P = -R
+    Rx = -(nu * (d2(u) - k2 * u) - lam_t * u - 1j * a_ * P)
+    Ry = -(nu * (d2(v) - k2 * v) - lam_t * v - _deriv_y(P))
+    Rz = -(nu * (d2(w) - k2 * w) - lam_t * w - 1j * g_ * P)
+    out = R.tausolve(kx, kz, LX, LZ, -1, 1, lam, nu, N, True, Rx, Ry, Rz)
+    uu, vv, ww, PP = out
+    div = 1j * a_ * uu + _deriv_y(vv) + 1j * g_ * ww
+    assert np.max(np.abs(div)) < 1e-9
+    for val in (uu, vv, ww):
+        assert abs(np.sum(val)) < 1e-12 and abs(np.sum(val * (-1) ** np.arange(N))) < 1e-12
+    # velocities recovered to the tau-method truncation level (tausolverTest's 1e-9)
+    for got, ref in ((uu, u), (vv, v), (ww, w)):
+        assert np.max(np.abs(got - ref)) < 1e-9 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("stepping", ["SBDF1", "SBDF3", "CNAB2", "SMRK2", "CNRK2"])
+def test_parabola_is_steady(stepping):
+    # dnsParabolaTest: Poiseuille parabola as base flow, zero fluctuation stays zero
+    g = R.Grid(8, 17, 8, LX, LZ)
+    fl = R.make_flags(baseflow="LaminarBase", constraint="PressureGradient", dPdx=-2 * 0.0025, nu=0.0025,
+                      timestepping=stepping, initstepping="SMRK2", dt=0.02, nonlinearity="Rotational")
+    d = R.OracleDNS(g, fl, g.zeros(3), g.zeros(1))
+    U, W = d.base()
+    y = np.cos(np.pi * np.arange(17) / 16)
+    assert np.max(np.abs(cheb_values(U, y) - (1 - y * y))) < 1e-14
+    d.advance(10)
+    u, _ = d.state()
+    assert np.sqrt(R.l2norm2(g, u)) < 1e-13
+
+
+def test_nonlinear_identities():
+    # skew = (conv + div)/2 holds exactly for the discretised operators on a
+    # field whose products are resolved; rotational = conv - grad(|u|^2/2) up to
+    # the pressure-like gradient, checked through the divergence-free part of v.
+    Nx, Ny, Nz = 24, 25, 24
+    g = R.Grid(Nx, Ny, Nz, LX, LZ)
+    u = random_field(Nx, Ny, Nz, LX, LZ, norm=0.2, decay=0.4, seed=5)
+    fl = dict(baseflow="ZeroBase", dealiasing="DealiasXZ")
+    f = {m: R.nonlinear(g, R.make_flags(nonlinearity=m, **fl), u)[0]
+         for m in ("Convection", "Divergence", "SkewSymmetric", "Rotational")}
+    skew = 0.5 * (f["Convection"] + f["Divergence"])
+    assert np.sqrt(R.l2dist2(g, f["SkewSymmetric"], skew) / R.l2norm2(g, skew)) < 1e-13
+    # conv - rot = grad(|u|^2 / 2): curl of the difference vanishes -> its x,z components satisfy
+    # i gamma d_x - i alpha d_z = 0 for each mode
+    dlt = f["Convection"] - f["Rotational"]
+    kx = np.fft.fftfreq(Nx, 1.0 / Nx)[None, :, None]
+    kz = np.arange(Nz // 2 + 1)[None, None, :]
+    al, ga = 2 * np.pi * kx / LX, 2 * np.pi * kz / LZ
+    wy = 1j * ga * dlt[0] - 1j * al * dlt[2]
+    assert np.max(np.abs(wy)) < 1e-12 * np.max(np.abs(f["Convection"]))
+
+
+def test_field_io_and_generator():
+    # the seeded generator is divergence-free, no-slip and normalised (oracle norms)
+    Nx, Ny, Nz = 32, 33, 32
+    g = R.Grid(Nx, Ny, Nz, LX, LZ)
+    u = random_field(Nx, Ny, Nz, LX, LZ, norm=0.1, decay=0.3, seed=7)
+    assert abs(np.sqrt(R.l2norm2(g, u)) - 0.1) < 1e-15
+    assert np.max(np.abs(divergence(u, LX, LZ))) < 1e-15
+    p = R.field_physical(g, u)
+    assert np.max(np.abs(p[:, 0])) < 1e-15 and np.max(np.abs(p[:, -1])) < 1e-15
+    assert np.max(np.abs(R.field_spectral(g, p) - u)) < 1e-15
