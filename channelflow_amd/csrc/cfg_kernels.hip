@@ -33,11 +33,57 @@ __device__ __forceinline__ double2 imul(double s, double2 a) { return make_doubl
 __device__ __forceinline__ int kx_of(const Geom& g, int ikx) { return ikx <= g.kxd ? ikx : ikx - g.nkx; }
 
 // ---------------------------------------------------------------------------
-// LDS Stockham FFT over nlines contiguous lines of length P.N. sign -1 forward
-// (exp(-i...)), +1 backward; unnormalised. Returns the buffer holding the
-// result (a or b). All threads of the block must call it.
+// LDS Stockham FFTs over nlines contiguous lines of length N. sign -1 forward
+// (exp(-i...)), +1 backward; unnormalised. tw is the table exp(-2 pi i j/N),
+// j < N, staged in LDS. The result lands in a or b (returned). All threads of
+// the block must call them.
 // ---------------------------------------------------------------------------
-__device__ double2* fft_lds(double2* a, double2* b, const FFTPlan& P, int nlines, int sign) {
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_small(double2 (&v)[4]) {
+    double2 o[4];
+    if constexpr (R == 2) {
+        o[0] = cadd(v[0], v[1]);
+        o[1] = csub(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        const double2 jd = (SIGN < 0) ? make_double2(d13.y, -d13.x) : make_double2(-d13.y, d13.x);
+        o[0] = cadd(s02, s13);
+        o[1] = cadd(d02, jd);
+        o[2] = csub(s02, s13);
+        o[3] = csub(d02, jd);
+    } else {
+        const double s3 = (SIGN < 0) ? -0.86602540378443864676 : 0.86602540378443864676;
+        const double2 s12 = cadd(v[1], v[2]), d12 = csub(v[1], v[2]);
+        const double2 m = make_double2(v[0].x - 0.5 * s12.x, v[0].y - 0.5 * s12.y);
+        const double2 q = make_double2(-s3 * d12.y, s3 * d12.x);
+        o[0] = cadd(v[0], s12);
+        o[1] = cadd(m, q);
+        o[2] = csub(m, q);
+    }
+    for (int r = 0; r < R; ++r) v[r] = o[r];
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void stockham_bfly(const double2* __restrict__ s, double2* __restrict__ d, const double2* tw,
+                                              int j, int k, int NR, int NS, int tstep) {
+    double2 v[4];
+    for (int r = 0; r < R; ++r) v[r] = s[j + r * NR];
+    if (k != 0) {
+        for (int r = 1; r < R; ++r) {
+            double2 t = tw[r * k * tstep];
+            if (SIGN > 0) t.y = -t.y;
+            v[r] = cmul(v[r], t);
+        }
+    }
+    dft_small<R, SIGN>(v);
+    double2* o = d + (j / NS) * NS * R + k;
+    for (int r = 0; r < R; ++r) o[r * NS] = v[r];
+}
+
+// runtime length (plan in P)
+template <int SIGN>
+__device__ double2* fft_lds(double2* a, double2* b, const FFTPlan& P, const double2* tw, int nlines) {
     double2* src = a;
     double2* dst = b;
     const int N = P.N;
@@ -50,40 +96,9 @@ __device__ double2* fft_lds(double2* a, double2* b, const FFTPlan& P, int nlines
             const int line = w / NR;
             const int j = w - line * NR;
             const int k = j % Ns;
-            const double2* s = src + line * N;
-            double2 v[4];
-            for (int r = 0; r < R; ++r) v[r] = s[j + r * NR];
-            if (k != 0) {
-                for (int r = 1; r < R; ++r) {
-                    double2 t = P.tw[r * k * tstep];
-                    if (sign > 0) t.y = -t.y;
-                    v[r] = cmul(v[r], t);
-                }
-            }
-            double2 o[4];
-            if (R == 2) {
-                o[0] = cadd(v[0], v[1]);
-                o[1] = csub(v[0], v[1]);
-            } else if (R == 4) {
-                const double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-                const double2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-                // (-i * sign) * d13 ... forward: -i d13, backward: +i d13
-                const double2 jd = (sign < 0) ? make_double2(d13.y, -d13.x) : make_double2(-d13.y, d13.x);
-                o[0] = cadd(s02, s13);
-                o[1] = cadd(d02, jd);
-                o[2] = csub(s02, s13);
-                o[3] = csub(d02, jd);
-            } else {  // R == 3
-                const double s3 = (sign < 0) ? -0.86602540378443864676 : 0.86602540378443864676;
-                const double2 s12 = cadd(v[1], v[2]), d12 = csub(v[1], v[2]);
-                const double2 m = make_double2(v[0].x - 0.5 * s12.x, v[0].y - 0.5 * s12.y);
-                const double2 q = make_double2(-s3 * d12.y, s3 * d12.x);
-                o[0] = cadd(v[0], s12);
-                o[1] = cadd(m, q);
-                o[2] = csub(m, q);
-            }
-            double2* d = dst + line * N + (j / Ns) * Ns * R + k;
-            for (int r = 0; r < R; ++r) d[r * Ns] = o[r];
+            if (R == 4) stockham_bfly<4, SIGN>(src + line * N, dst + line * N, tw, j, k, NR, Ns, tstep);
+            else if (R == 2) stockham_bfly<2, SIGN>(src + line * N, dst + line * N, tw, j, k, NR, Ns, tstep);
+            else stockham_bfly<3, SIGN>(src + line * N, dst + line * N, tw, j, k, NR, Ns, tstep);
         }
         __syncthreads();
         double2* t = src;
@@ -92,6 +107,65 @@ __device__ double2* fft_lds(double2* a, double2* b, const FFTPlan& P, int nlines
         Ns *= R;
     }
     return src;
+}
+
+// compile-time length: every divisor is a constant
+constexpr int fft_radix(int n, int p) {
+    int c = 0;
+    while (n % 4 == 0) {
+        if (c == p) return 4;
+        n /= 4;
+        ++c;
+    }
+    while (n % 2 == 0) {
+        if (c == p) return 2;
+        n /= 2;
+        ++c;
+    }
+    while (n % 3 == 0) {
+        if (c == p) return 3;
+        n /= 3;
+        ++c;
+    }
+    return 0;
+}
+constexpr int fft_npass(int n) {
+    int c = 0;
+    while (fft_radix(n, c) != 0) ++c;
+    return c;
+}
+constexpr int fft_ns(int n, int p) {
+    int s = 1;
+    for (int q = 0; q < p; ++q) s *= fft_radix(n, q);
+    return s;
+}
+
+template <int N, int SIGN, int P>
+__device__ __forceinline__ double2* fft_ct(double2* a, double2* b, const double2* tw, int nlines) {
+    if constexpr (P == fft_npass(N)) {
+        return a;
+    } else {
+        constexpr int R = fft_radix(N, P), NR = N / R, NS = fft_ns(N, P), TSTEP = N / (NS * R);
+        for (int w = threadIdx.x; w < nlines * NR; w += blockDim.x) {
+            const int line = w / NR;
+            const int j = w - line * NR;
+            stockham_bfly<R, SIGN>(a + line * N, b + line * N, tw, j, j % NS, NR, NS, TSTEP);
+        }
+        __syncthreads();
+        return fft_ct<N, SIGN, P + 1>(b, a, tw, nlines);
+    }
+}
+
+// N == 0 selects the runtime-length path
+template <int N, int SIGN>
+__device__ __forceinline__ double2* fft_any(double2* a, double2* b, const FFTPlan& P, const double2* tw, int nlines) {
+    if constexpr (N == 0) return fft_lds<SIGN>(a, b, P, tw, nlines);
+    else return fft_ct<N, SIGN, 0>(a, b, tw, nlines);
+}
+
+// stage the twiddle table of P into LDS
+__device__ __forceinline__ void stage_tw(double2* dst, const FFTPlan& P) {
+    for (int i = threadIdx.x; i < P.N; i += blockDim.x) dst[i] = P.tw[i];
 }
 
 // ===========================================================================
@@ -511,21 +585,23 @@ __global__ void __launch_bounds__(64) k_yinv(Geom g, YMats ym, const double2* __
 }
 
 // inverse x-FFT: yp [F][Ny][nkz][nkx] -> xp [Fo][Ny][Nx][nkz]; output f >= nsrc is
-// d/dx of source field f - nsrc
-constexpr int XLB = 8;  // lines (kz values) per block in the x-FFT kernels
-
-__global__ void __launch_bounds__(256) k_xinv(Geom g, FFTPlan px, int nsrc, const double2* __restrict__ yp,
+// d/dx of source field f - nsrc. XL kz-lines per block.
+template <int NX>
+__global__ void __launch_bounds__(256) k_xinv(Geom g, FFTPlan px, int XL, int nsrc, const double2* __restrict__ yp,
                                               double2* __restrict__ xp) {
     extern __shared__ double2 lds[];
     const int Nx = g.Nx, nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
-    const int nchunk = (nkz + XLB - 1) / XLB;
-    const int y = blockIdx.x / nchunk, ikz0 = (blockIdx.x % nchunk) * XLB;
+    const int nchunk = (nkz + XL - 1) / XL;
+    const int y = blockIdx.x / nchunk, ikz0 = (blockIdx.x % nchunk) * XL;
     const int fo = blockIdx.y;
     const bool deriv = fo >= nsrc;
     const int fs = deriv ? fo - nsrc : fo;
-    const int nl = min(XLB, nkz - ikz0);
-    double2* a = lds;
-    double2* b = lds + XLB * Nx;
+    const int nl = min(XL, nkz - ikz0);
+    double2* tw = lds;
+    double2* a = lds + Nx;
+    double2* b = a + XL * Nx;
+    stage_tw(tw, px);
+    const double dk = 2 * PI / g.Lx;
     for (int w = threadIdx.x; w < nl * Nx; w += blockDim.x) {
         const int l = w / Nx, j = w - l * Nx;
         int ikx = -1;
@@ -534,40 +610,45 @@ __global__ void __launch_bounds__(256) k_xinv(Geom g, FFTPlan px, int nsrc, cons
         double2 v = make_double2(0.0, 0.0);
         if (ikx >= 0) {
             v = yp[(((long)fs * Ny + y) * nkz + ikz0 + l) * nkx + ikx];
-            if (deriv) v = imul(2 * PI * kx_of(g, ikx) / g.Lx, v);
+            if (deriv) v = imul(dk * kx_of(g, ikx), v);
         }
         a[l * Nx + j] = v;
     }
     __syncthreads();
-    double2* r = fft_lds(a, b, px, nl, +1);
+    double2* r = fft_any<NX, +1>(a, b, px, tw, nl);
+    double2* out = xp + (((long)fo * Ny + y) * Nx) * nkz + ikz0;
     for (int w = threadIdx.x; w < nl * Nx; w += blockDim.x) {
         const int x = w / nl, l = w - x * nl;
-        xp[(((long)fo * Ny + y) * Nx + x) * nkz + ikz0 + l] = r[l * Nx + x];
+        out[(long)x * nkz + l] = r[l * Nx + x];
     }
 }
 
 // forward x-FFT: zo [G][Ny][Nx][nkz] -> fo [G][Ny][nkz][nkx], scaled by 1/(Nx Nz)
-__global__ void __launch_bounds__(256) k_xfwd(Geom g, FFTPlan px, const double2* __restrict__ zo,
+template <int NX>
+__global__ void __launch_bounds__(256) k_xfwd(Geom g, FFTPlan px, int XL, const double2* __restrict__ zo,
                                               double2* __restrict__ fo) {
     extern __shared__ double2 lds[];
     const int Nx = g.Nx, nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
-    const int nchunk = (nkz + XLB - 1) / XLB;
-    const int y = blockIdx.x / nchunk, ikz0 = (blockIdx.x % nchunk) * XLB;
+    const int nchunk = (nkz + XL - 1) / XL;
+    const int y = blockIdx.x / nchunk, ikz0 = (blockIdx.x % nchunk) * XL;
     const int f = blockIdx.y;
-    const int nl = min(XLB, nkz - ikz0);
-    double2* a = lds;
-    double2* b = lds + XLB * Nx;
+    const int nl = min(XL, nkz - ikz0);
+    double2* tw = lds;
+    double2* a = lds + Nx;
+    double2* b = a + XL * Nx;
+    stage_tw(tw, px);
+    const double2* in = zo + (((long)f * Ny + y) * Nx) * nkz + ikz0;
     for (int w = threadIdx.x; w < nl * Nx; w += blockDim.x) {
         const int x = w / nl, l = w - x * nl;
-        a[l * Nx + x] = zo[(((long)f * Ny + y) * Nx + x) * nkz + ikz0 + l];
+        a[l * Nx + x] = in[(long)x * nkz + l];
     }
     __syncthreads();
-    double2* r = fft_lds(a, b, px, nl, -1);
+    double2* r = fft_any<NX, -1>(a, b, px, tw, nl);
     const double scale = 1.0 / ((double)g.Nx * g.Nz);
     for (int w = threadIdx.x; w < nl * nkx; w += blockDim.x) {
         const int l = w / nkx, ikx = w - l * nkx;
         const int j = ikx <= g.kxd ? ikx : ikx - nkx + Nx;
-        double2 v = r[l * Nx + j];
+        const double2 v = r[l * Nx + j];
         fo[(((long)f * Ny + y) * nkz + ikz0 + l) * nkx + ikx] = make_double2(v.x * scale, v.y * scale);
     }
 }
@@ -577,7 +658,7 @@ struct ZCtx {
     Geom g;
     FFTPlan pz;
     int y, x0, nl;
-    double2 *wa, *wb;
+    double2 *wa, *wb, *tw;
     double* phys;  // [NP][XB][Nz]
     int XB;
 };
@@ -586,44 +667,51 @@ __device__ __forceinline__ double* physf(const ZCtx& c, int f) { return c.phys +
 
 // c2r of two real fields at once: field fa -> phys[da], fb -> phys[db]
 // (fb < 0: none). dza/dzb apply d/dz. Imag parts of kz = 0 are ignored, as
-// FFTW's c2r does.
+// FFTW's c2r does. The unused middle bins are zeroed in the same sweep.
+template <int NZ>
 __device__ void zc2r_pair(const ZCtx& c, const double2* __restrict__ xp, int fa, bool dza, int da, int fb, bool dzb,
                           int db) {
     const Geom& g = c.g;
     const int Nz = g.Nz, nkz = g.nkz;
-    for (int w = threadIdx.x; w < c.nl * Nz; w += blockDim.x) c.wa[w] = make_double2(0.0, 0.0);
-    __syncthreads();
+    const int nmid = Nz - 2 * nkz + 1;  // bins nkz .. Nz-nkz
+    for (int w = threadIdx.x; w < c.nl * nmid; w += blockDim.x) {
+        const int l = w / nmid, j = w - l * nmid;
+        c.wa[l * Nz + nkz + j] = make_double2(0.0, 0.0);
+    }
+    const long fsz = (long)g.Ny * g.Nx * nkz;
+    const double dk = 2 * PI / g.Lz;
     for (int w = threadIdx.x; w < c.nl * nkz; w += blockDim.x) {
         const int l = w / nkz, k = w - l * nkz;
         const long base = ((long)c.y * g.Nx + c.x0 + l) * nkz + k;
-        const double s = 2 * PI * k / g.Lz;
-        double2 A = xp[(long)fa * g.Ny * g.Nx * nkz + base];
-        if (dza) A = imul(s, A);
+        double2 A = xp[(long)fa * fsz + base];
+        if (dza) A = imul(dk * k, A);
         double2 B = make_double2(0.0, 0.0);
         if (fb >= 0) {
-            B = xp[(long)fb * g.Ny * g.Nx * nkz + base];
-            if (dzb) B = imul(s, B);
+            B = xp[(long)fb * fsz + base];
+            if (dzb) B = imul(dk * k, B);
         }
         double2* z = c.wa + l * Nz;
         if (k == 0) {
             z[0] = make_double2(A.x, B.x);
         } else {
-            z[k] = make_double2(A.x - B.y, A.y + B.x);         // A + iB
-            z[Nz - k] = make_double2(A.x + B.y, B.x - A.y);    // conj(A) + i conj(B)
+            z[k] = make_double2(A.x - B.y, A.y + B.x);       // A + iB
+            z[Nz - k] = make_double2(A.x + B.y, B.x - A.y);  // conj(A) + i conj(B)
         }
     }
     __syncthreads();
-    double2* r = fft_lds(c.wa, c.wb, c.pz, c.nl, +1);
+    double2* r = fft_any<NZ, +1>(c.wa, c.wb, c.pz, c.tw, c.nl);
     double* pa = physf(c, da);
     double* pb = db >= 0 ? physf(c, db) : nullptr;
     for (int w = threadIdx.x; w < c.nl * Nz; w += blockDim.x) {
-        pa[w] = r[w].x;
-        if (pb) pb[w] = r[w].y;
+        const double2 v = r[w];
+        pa[w] = v.x;
+        if (pb) pb[w] = v.y;
     }
     __syncthreads();
 }
 
 // r2c of phys[sa] (and phys[sb]) -> zo fields ga, gb (kz <= kzd kept)
+template <int NZ>
 __device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int ga, int sb, int gb) {
     const Geom& g = c.g;
     const int Nz = g.Nz, nkz = g.nkz;
@@ -631,15 +719,15 @@ __device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int g
     const double* pb = sb >= 0 ? physf(c, sb) : nullptr;
     for (int w = threadIdx.x; w < c.nl * Nz; w += blockDim.x) c.wa[w] = make_double2(pa[w], pb ? pb[w] : 0.0);
     __syncthreads();
-    double2* r = fft_lds(c.wa, c.wb, c.pz, c.nl, -1);
+    double2* r = fft_any<NZ, -1>(c.wa, c.wb, c.pz, c.tw, c.nl);
+    const long fsz = (long)g.Ny * g.Nx * nkz;
     for (int w = threadIdx.x; w < c.nl * nkz; w += blockDim.x) {
         const int l = w / nkz, k = w - l * nkz;
         const double2 Zk = r[l * Nz + k];
         const double2 Zn = r[l * Nz + ((Nz - k) % Nz)];
         const long base = ((long)c.y * g.Nx + c.x0 + l) * nkz + k;
-        zo[(long)ga * g.Ny * g.Nx * nkz + base] = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
-        if (gb >= 0)
-            zo[(long)gb * g.Ny * g.Nx * nkz + base] = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
+        zo[(long)ga * fsz + base] = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
+        if (gb >= 0) zo[(long)gb * fsz + base] = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
     }
     __syncthreads();
 }
@@ -648,7 +736,7 @@ __device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int g
 // du/dy dv/dy dw/dy du/dx dv/dx dw/dx; DIV u v w. zo outputs: g0..g2 (the
 // u.grad u / omega x u part and the rotation term) then uu00 uu01 uu02 uu11
 // uu12 uu22 for DIV/SKEW.
-template <int METHOD>
+template <int METHOD, int NZ>
 __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, double rotation, int want_g,
                                                const double2* __restrict__ xp, double2* __restrict__ zo) {
     extern __shared__ double2 lds[];
@@ -660,15 +748,17 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, doubl
     c.x0 = (blockIdx.x % nxc) * XB;
     c.nl = min(XB, g.Nx - c.x0);
     c.XB = XB;
-    c.wa = lds;
-    c.wb = lds + XB * g.Nz;
-    c.phys = reinterpret_cast<double*>(lds + 2 * XB * g.Nz);
+    c.tw = lds;
+    c.wa = lds + g.Nz;
+    c.wb = c.wa + XB * g.Nz;
+    c.phys = reinterpret_cast<double*>(c.wb + XB * g.Nz);
+    stage_tw(c.tw, pz);
     const int Nz = g.Nz;
     const int npts = c.nl * Nz;
     if (METHOD == NL_ROT) {
-        zc2r_pair(c, xp, 0, false, 0, 1, false, 1);
-        zc2r_pair(c, xp, 2, false, 2, 3, false, 3);
-        zc2r_pair(c, xp, 4, false, 4, 5, false, 5);
+        zc2r_pair<NZ>(c, xp, 0, false, 0, 1, false, 1);
+        zc2r_pair<NZ>(c, xp, 2, false, 2, 3, false, 3);
+        zc2r_pair<NZ>(c, xp, 4, false, 4, 5, false, 5);
         double *u0 = physf(c, 0), *u1 = physf(c, 1), *u2 = physf(c, 2);
         double *w0 = physf(c, 3), *w1 = physf(c, 4), *w2 = physf(c, 5);
         for (int w = threadIdx.x; w < npts; w += blockDim.x) {  // cross(vort, u) (diffops.cpp:2598-2605)
@@ -685,13 +775,13 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, doubl
             w2[w] = f2;
         }
         __syncthreads();
-        zr2c_pair(c, zo, 3, 0, 4, 1);
-        zr2c_pair(c, zo, 5, 2, -1, -1);
+        zr2c_pair<NZ>(c, zo, 3, 0, 4, 1);
+        zr2c_pair<NZ>(c, zo, 5, 2, -1, -1);
         return;
     }
     // u, v, w -> phys 0..2
-    zc2r_pair(c, xp, 0, false, 0, 1, false, 1);
-    zc2r_pair(c, xp, 2, false, 2, -1, false, -1);
+    zc2r_pair<NZ>(c, xp, 0, false, 0, 1, false, 1);
+    zc2r_pair<NZ>(c, xp, 2, false, 2, -1, false, -1);
     if (METHOD == NL_CONV || METHOD == NL_SKEW) {
         // f_i = sum_j (s u_j) du_i/dx_j accumulated over j = x, y, z
         // (diffops.cpp:3192-3199, 3626-3633); phys 3..5 = f, 6..8 = du_i/dx_j
@@ -701,14 +791,14 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, doubl
         double *t0 = physf(c, 6), *t1 = physf(c, 7), *t2 = physf(c, 8);
         for (int j = 0; j < 3; ++j) {
             if (j == 0) {  // d/dx from xp 6..8
-                zc2r_pair(c, xp, 6, false, 6, 7, false, 7);
-                zc2r_pair(c, xp, 8, false, 8, -1, false, -1);
+                zc2r_pair<NZ>(c, xp, 6, false, 6, 7, false, 7);
+                zc2r_pair<NZ>(c, xp, 8, false, 8, -1, false, -1);
             } else if (j == 1) {  // d/dy from xp 3..5
-                zc2r_pair(c, xp, 3, false, 6, 4, false, 7);
-                zc2r_pair(c, xp, 5, false, 8, -1, false, -1);
+                zc2r_pair<NZ>(c, xp, 3, false, 6, 4, false, 7);
+                zc2r_pair<NZ>(c, xp, 5, false, 8, -1, false, -1);
             } else {  // d/dz of u v w
-                zc2r_pair(c, xp, 0, true, 6, 1, true, 7);
-                zc2r_pair(c, xp, 2, true, 8, -1, false, -1);
+                zc2r_pair<NZ>(c, xp, 0, true, 6, 1, true, 7);
+                zc2r_pair<NZ>(c, xp, 2, true, 8, -1, false, -1);
             }
             const double* uj = (j == 0) ? u0 : (j == 1) ? u1 : u2;
             for (int w = threadIdx.x; w < npts; w += blockDim.x) {
@@ -732,8 +822,8 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, doubl
             }
             __syncthreads();
         }
-        zr2c_pair(c, zo, 3, 0, 4, 1);
-        zr2c_pair(c, zo, 5, 2, -1, -1);
+        zr2c_pair<NZ>(c, zo, 3, 0, 4, 1);
+        zr2c_pair<NZ>(c, zo, 5, 2, -1, -1);
     } else if (want_g) {  // DIV with rotation: g = rotation terms only
         double *u0 = physf(c, 0), *u1 = physf(c, 1);
         double *f0 = physf(c, 3), *f1 = physf(c, 4), *f2 = physf(c, 5);
@@ -743,8 +833,8 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, doubl
             f2[w] = 0.0;
         }
         __syncthreads();
-        zr2c_pair(c, zo, 3, 0, 4, 1);
-        zr2c_pair(c, zo, 5, 2, -1, -1);
+        zr2c_pair<NZ>(c, zo, 3, 0, 4, 1);
+        zr2c_pair<NZ>(c, zo, 5, 2, -1, -1);
     }
     if (METHOD == NL_DIV || METHOD == NL_SKEW) {  // outer(u, u) (diffops.cpp:2336-2388)
         double *u0 = physf(c, 0), *u1 = physf(c, 1), *u2 = physf(c, 2);
@@ -760,9 +850,9 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, doubl
             p8[w] = a2 * a2;
         }
         __syncthreads();
-        zr2c_pair(c, zo, 3, 3, 4, 4);
-        zr2c_pair(c, zo, 5, 5, 6, 6);
-        zr2c_pair(c, zo, 7, 7, 8, 8);
+        zr2c_pair<NZ>(c, zo, 3, 3, 4, 4);
+        zr2c_pair<NZ>(c, zo, 5, 5, 6, 6);
+        zr2c_pair<NZ>(c, zo, 7, 7, 8, 8);
     }
 }
 
@@ -825,11 +915,80 @@ __global__ void __launch_bounds__(64) k_yfwd(Geom g, YMats ym, int have_g, const
     }
 }
 
+// LDS budget per block (set from the device's opt-in limit at start-up)
+static size_t g_lds_budget = 64 * 1024;
+void set_lds_budget(size_t bytes) { g_lds_budget = bytes; }
+
 static int zpass_xb(const Geom& g, int nphys) {
     const long per = (long)g.Nz * (32 + 8 * nphys);  // two complex work lines + nphys real lines
+    const long avail = (long)g_lds_budget - 16L * g.Nz;
     int xb = 1;
-    while (xb < 32 && xb * 2 <= g.Nx && (long)(xb * 2) * per <= 64 * 1024) xb *= 2;
+    while (xb < 32 && xb * 2 <= g.Nx && (long)(xb * 2) * per <= avail) xb *= 2;
     return xb;
+}
+
+static int xfft_lines(const Geom& g) {
+    const long per = 32L * g.Nx;  // two complex work lines
+    const long avail = (long)g_lds_budget - 16L * g.Nx;
+    int xl = 4;
+    while (xl < 16 && (long)(xl * 2) * per <= avail) xl *= 2;
+    return xl;
+}
+
+template <class K>
+static void allow_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+#define CFG_FFT_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+
+template <int NX>
+static void launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, const double2* in,
+                     double2* out, hipStream_t st) {
+    const int XL = xfft_lines(g);
+    const int nchunk = (g.nkz + XL - 1) / XL;
+    const size_t lds = (size_t)(g.Nx + 2 * XL * g.Nx) * sizeof(double2);
+    if (inverse) {
+        allow_lds(k_xinv<NX>, lds);
+        k_xinv<NX><<<dim3(g.Ny * nchunk, nf), 256, lds, st>>>(g, px, XL, nsrc, in, out);
+    } else {
+        allow_lds(k_xfwd<NX>, lds);
+        k_xfwd<NX><<<dim3(g.Ny * nchunk, nf), 256, lds, st>>>(g, px, XL, in, out);
+    }
+}
+
+static void launch_xfft(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, const double2* in,
+                        double2* out, hipStream_t st) {
+    switch (g.Nx) {
+#define CASE_X(N) \
+    case N: launch_x<N>(inverse, g, px, nsrc, nf, in, out, st); return;
+        CFG_FFT_SIZES(CASE_X)
+#undef CASE_X
+        default: launch_x<0>(inverse, g, px, nsrc, nf, in, out, st);
+    }
+}
+
+template <int METHOD, int NZ>
+static void launch_z_t(const Geom& g, const FFTPlan& pz, double rotation, int want_g, const double2* xp,
+                       double2* zo, hipStream_t st) {
+    const int nphys = (METHOD == NL_ROT) ? 6 : 9;
+    const int XB = zpass_xb(g, nphys);
+    const size_t lds = (size_t)g.Nz * sizeof(double2) + (size_t)XB * g.Nz * (2 * sizeof(double2) + nphys * sizeof(double));
+    const int grid = g.Ny * ((g.Nx + XB - 1) / XB);
+    allow_lds(k_zpass<METHOD, NZ>, lds);
+    k_zpass<METHOD, NZ><<<grid, 256, lds, st>>>(g, pz, XB, rotation, want_g, xp, zo);
+}
+
+template <int METHOD>
+static void launch_z(const Geom& g, const FFTPlan& pz, double rotation, int want_g, const double2* xp, double2* zo,
+                     hipStream_t st) {
+    switch (g.Nz) {
+#define CASE_Z(N) \
+    case N: launch_z_t<METHOD, N>(g, pz, rotation, want_g, xp, zo, st); return;
+        CFG_FFT_SIZES(CASE_Z)
+#undef CASE_Z
+        default: launch_z_t<METHOD, 0>(g, pz, rotation, want_g, xp, zo, st);
+    }
 }
 
 void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, double Vsuck,
@@ -843,8 +1002,6 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     double2* zo = reinterpret_cast<double2*>(w.zo);
     double2* fo = reinterpret_cast<double2*>(w.fo);
     double2* f2 = reinterpret_cast<double2*>(f);
-    const int nchunk = (g.nkz + XLB - 1) / XLB;
-    const size_t xlds = 2 * XLB * g.Nx * sizeof(double2);
     int nsrc = 3, nxout = 3;
     hipEvent_t ev = T.begin();
     switch (method) {
@@ -867,28 +1024,24 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     T.end(K_YINV, ev);
     ev = T.begin();
     // d/dx outputs (fields nsrc..) derive from fields 0..2 only
-    k_xinv<<<dim3(g.Ny * nchunk, nxout), 256, xlds, st>>>(g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, yp, xp);
+    launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, yp, xp, st);
     T.end(K_XINV, ev);
-    const int nphys = (method == NL_ROT) ? 6 : 9;
-    const int XB = zpass_xb(g, nphys);
-    const size_t zlds = (size_t)XB * g.Nz * (2 * sizeof(double2) + nphys * sizeof(double));
-    const int zgrid = g.Ny * ((g.Nx + XB - 1) / XB);
     const int want_g = (method != NL_DIV) || (rotation != 0.0);
     ev = T.begin();
     switch (method) {
-        case NL_ROT: k_zpass<NL_ROT><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo); break;
-        case NL_CONV: k_zpass<NL_CONV><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo); break;
-        case NL_SKEW: k_zpass<NL_SKEW><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo); break;
-        default: k_zpass<NL_DIV><<<zgrid, 256, zlds, st>>>(g, pz, XB, rotation, want_g, xp, zo);
+        case NL_ROT: launch_z<NL_ROT>(g, pz, rotation, want_g, xp, zo, st); break;
+        case NL_CONV: launch_z<NL_CONV>(g, pz, rotation, want_g, xp, zo, st); break;
+        case NL_SKEW: launch_z<NL_SKEW>(g, pz, rotation, want_g, xp, zo, st); break;
+        default: launch_z<NL_DIV>(g, pz, rotation, want_g, xp, zo, st);
     }
     T.end(K_ZPASS, ev);
     ev = T.begin();
     // forward x-FFT of the z-pass outputs that exist
-    if (want_g) k_xfwd<<<dim3(g.Ny * nchunk, 3), 256, xlds, st>>>(g, px, zo, fo);
+    if (want_g) launch_xfft(false, g, px, 0, 3, zo, fo, st);
     if (method == NL_DIV || method == NL_SKEW) {
         const long off = 3L * g.Ny * g.Nx * g.nkz;
         const long offo = 3L * g.Ny * g.M;
-        k_xfwd<<<dim3(g.Ny * nchunk, 6), 256, xlds, st>>>(g, px, zo + off, fo + offo);
+        launch_xfft(false, g, px, 0, 6, zo + off, fo + offo, st);
     }
     T.end(K_XFWD, ev);
     ev = T.begin();

@@ -144,6 +144,9 @@ void launch_linear(const Geom& g, double nu, const LinearConst& c, const double*
                    double* scratch, hipStream_t st);
 
 // ---- nonlinear term ---------------------------------------------------------
+// LDS bytes a block may use (the device's opt-in limit); default 64 KiB.
+void set_lds_budget(size_t bytes);
+
 struct NLWork {
     double* yp;  // [6][Ny][M] complex
     double* xp;  // [9][Ny][Nx][nkz] complex

@@ -354,6 +354,16 @@ struct Context {
     KTimer kt;
 
     Context(const cfg_grid& gr, bool dealias) {
+        static bool lds_set = false;
+        if (!lds_set) {  // largest dynamic LDS a block may opt into on this device
+            int dev = 0;
+            HIPCHK(hipGetDevice(&dev));
+            hipDeviceProp_t p;
+            HIPCHK(hipGetDeviceProperties(&p, dev));
+            size_t b = p.sharedMemPerBlockOptin ? p.sharedMemPerBlockOptin : p.sharedMemPerBlock;
+            set_lds_budget(std::min<size_t>(b, 160 * 1024));
+            lds_set = true;
+        }
         HIPCHK(hipStreamCreate(&st));
         kt.st = st;
         geo.build(gr, dealias);

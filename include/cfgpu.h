@@ -1,10 +1,9 @@
 /* cfgpu -- C ABI of the MI355X-native channelflow DNS time step.
  *
  * This is the drop-in boundary for channelflow's DNS::advance path. Host code
- * keeps channelflow's FlowField / DNS / DNSFlags interface (see
- * include/cfgpu_dns.hpp for the C++ facade) and calls the GPU through these
- * entry points only: plain pointers and sizes, integer status codes, no
- * exit() inside the library.
+ * keeps channelflow's FlowField / DNS / DNSFlags interface and calls the GPU
+ * through these entry points only: plain pointers and sizes, integer status
+ * codes, no exit() inside the library.
  *
  * Spectral field buffers passed across this boundary use channelflow's serial
  * complex layout (channelflow/flowfield.h:387-401), interleaved re/im doubles:
