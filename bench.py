@@ -95,6 +95,78 @@ def cpu_baseline(wl, steps_timed, budget_s=30.0):
                        f"then {n} SBDF3 steps timed ({dt:.1f} s) on 1 core")
 
 
+COPY_CAL_BYTES = 1 << 28
+
+
+def copy_calibration(nbytes=COPY_CAL_BYTES, reps=3):
+    """Device-to-device copies of nbytes by the HIP runtime's blit kernel. A
+    profiled run reads its FETCH_SIZE / WRITE_SIZE counters against the known
+    nbytes read and written per copy to scale the counters of the DNS kernels."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+
+    def chk(rc, what):
+        if rc != 0:
+            raise SystemExit(f"copy calibration: {what} failed ({rc})")
+
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    chk(hip.hipMalloc(ctypes.byref(a), nbytes), "hipMalloc")
+    chk(hip.hipMalloc(ctypes.byref(b), nbytes), "hipMalloc")
+    chk(hip.hipMemset(a, 1, nbytes), "hipMemset")
+    for _ in range(reps):
+        chk(hip.hipMemcpy(b, a, nbytes, 3), "hipMemcpy")   # 3 = hipMemcpyDeviceToDevice
+    chk(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+    chk(hip.hipFree(a), "hipFree")
+    chk(hip.hipFree(b), "hipFree")
+    print(f"copy calibration: {reps} device copies of {nbytes} bytes", flush=True)
+
+
+def _counter_rows(path, counter):
+    """(short kernel name, value) per dispatch from a rocprofv3 counter_collection.csv."""
+    import csv
+    import re
+    rows = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"]
+            m = re.search(r"cfg::(k_\w+?)(?:_wave)?(?:<|\()", name)
+            rows.append((m.group(1) if m else name.split("(")[0].strip(), float(r["Counter_Value"])))
+    return rows
+
+
+def traffic_from_counters(fetch_csv, write_csv, out_path, command):
+    """HBM bytes per launch of each DNS kernel from one FETCH_SIZE and one
+    WRITE_SIZE run (KiB per dispatch), each scaled by the same counter's reading
+    on the plain copies of copy_calibration()."""
+    per = {}
+    scale = {}
+    for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
+        rows = _counter_rows(path, counter)
+        cal = [v for n, v in rows if "copyBuffer" in n and v * 1024 > COPY_CAL_BYTES / 2]
+        if not cal:
+            raise SystemExit(f"{path}: no calibration copy dispatches of {COPY_CAL_BYTES} bytes")
+        scale[key] = COPY_CAL_BYTES / (1024.0 * sum(cal) / len(cal))
+        acc = {}
+        for n, v in rows:
+            if n.startswith("k_"):
+                acc.setdefault(n, []).append(v)
+        for n, vs in acc.items():
+            per.setdefault(n, {})[key] = 1024.0 * scale[key] * sum(vs) / len(vs)
+    out = {n: d.get("fetch", 0.0) + d.get("write", 0.0) for n, d in per.items()}
+    out["_per_kernel"] = per
+    out["_scale"] = scale
+    out["_command"] = command
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps({k: v for k, v in out.items() if not k.startswith("_")}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,7 +175,17 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run plain device copies of known size (for scaling HBM counters)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic_<workload>.json from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+    if args.traffic_from:
+        cmd = ("rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -f csv -- python bench.py --workload "
+               f"{args.workload} --steps 5 --warmup 3 --no-cpu --copy-calibration")
+        traffic_from_counters(*args.traffic_from, os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json"),
+                              cmd)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(ensembles shard by member across GPUs; see the sharding notes of the design)", file=sys.stderr)
@@ -166,6 +248,8 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
+    if args.copy_calibration:
+        copy_calibration()
     print(json.dumps(out), flush=True)
 
 

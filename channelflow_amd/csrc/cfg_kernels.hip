@@ -226,7 +226,7 @@ __device__ __forceinline__ CV cview(double* base, const Geom& g, int comp, int m
 constexpr int TAU_WPB = 8;  // waves (modes) per block; 8 complex doubles = one 128-byte row
 
 template <int K>
-__device__ void tau_P_and_v_wave(const WCtx& c, const double* cm, const double* sc, double kappa2, double lambda,
+__device__ __forceinline__ void tau_P_and_v_wave(const WCtx& c, const double* cm, const double* sc, double kappa2, double lambda,
                                  double nu, bool taucorr, const WV<K>& r, const WV<K>& Ry, WV<K>& P, WV<K>& v) {
     const long Ny = c.N;
     P = whelm<K>(r, cm + 0 * Ny, cm + 1 * Ny, cm + 2 * Ny, kappa2, 0.0, 0.0, c);
@@ -253,9 +253,9 @@ __device__ void tau_P_and_v_wave(const WCtx& c, const double* cm, const double* 
     }
     if (!taucorr) return;
     const int Nb = c.Nb;
-    double sigma1_Nb = lambda * wget<K>(v, Nb, c) - nu * 0.0 - wget<K>(Ry, Nb, c);
-    double sigma1_Nb1 = lambda * wget<K>(v, Nb - 1, c) - nu * 0.0 - wget<K>(Ry, Nb - 1, c);
-    sigma1_Nb1 += (4.0 / c.L) * Nb * wget<K>(P, Nb, c);
+    double sigma1_Nb = lambda * wtop<K>(v, c) - nu * 0.0 - wtop<K>(Ry, c);
+    double sigma1_Nb1 = lambda * wtop1<K>(v, c) - nu * 0.0 - wtop1<K>(Ry, c);
+    sigma1_Nb1 += (4.0 / c.L) * Nb * wtop<K>(P, c);
     const double sigmaNb = sigma1_Nb / (1.0 - sc[5]);
     const double sigmaNb1 = sigma1_Nb1 / (1.0 - sc[4]);
     const WV<K> P0 = wload<K>(cm + 10 * Ny, c), v0 = wload<K>(cm + 11 * Ny, c);
@@ -369,10 +369,10 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, TauSet 
         const double* vb = cm + 3L * Ny;
         const double* vu = cm + 4L * Ny;
         const double* vd = cm + 5L * Ny;
-        for (int comp = 0; comp < 3; comp += 2) {
-            const double k2 = comp == 0 ? kx2pi : kz2pi;
-            const WV<K>& Rr = comp == 0 ? Rxr : Rzr;
-            const WV<K>& Ri = comp == 0 ? Rxi : Rzi;
+        // u (slot 0) and w (slot 1) from the velocity Helmholtz operator. Called
+        // with explicit register arrays: selecting between them at run time
+        // would put them in scratch memory.
+        auto solve_uw = [&](double k2, const WV<K>& Rr, const WV<K>& Ri, double umean, double* mu_out, int slot) {
             WV<K> fr, fi;
             for (int k = 0; k < K; ++k) {
                 fr.e[k] = -(k2 * Pi.e[k]) - Rr.e[k];
@@ -382,14 +382,16 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, TauSet 
             }
             WV<K> xr, xi;
             if (zero && zc.bulk) {  // bulk-velocity constraint (tausolver.cpp:404-450)
-                const double mu = whelm_mean<K>(fr, vb, vu, vd, lambda, nu, comp == 0 ? zc.umean : zc.wmean, c, xr);
-                if (c.lane == 0) *(comp == 0 ? zc.dPdx_out : zc.dPdz_out) = mu;
+                const double mu = whelm_mean<K>(fr, vb, vu, vd, lambda, nu, umean, c, xr);
+                if (c.lane == 0) *mu_out = mu;
             } else {
                 xr = whelm<K>(fr, vb, vu, vd, lambda, 0.0, 0.0, c);
             }
             xi = zero ? wzero<K>() : whelm<K>(fi, vb, vu, vd, lambda, 0.0, 0.0, c);
-            wstore_c<K>(lds + (w * 3 + (comp == 0 ? 0 : 1)) * Ny, c, xr, xi);
-        }
+            wstore_c<K>(lds + (w * 3 + slot) * Ny, c, xr, xi);
+        };
+        solve_uw(kx2pi, Rxr, Rxi, zc.umean, zc.dPdx_out, 0);
+        solve_uw(kz2pi, Rzr, Rzi, zc.wmean, zc.dPdz_out, 1);
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < 2 * Ny * TAU_WPB; idx += nthr) {

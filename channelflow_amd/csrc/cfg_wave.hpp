@@ -82,14 +82,26 @@ __device__ __forceinline__ void wstore_s(double* p, long s, const WV<K>& v, cons
 }
 
 // coefficient n of v, broadcast to all lanes
+// Blocked entry i of one parity array, broadcast to all lanes. Each candidate
+// passes through an empty asm operand: without it the select chain is folded
+// into an indexed load, which puts the array in scratch memory.
 template <int K>
-__device__ __forceinline__ double wget(const WV<K>& v, int n, const WCtx& c) {
-    const int i = n >> 1, owner = i / K, slot = i - owner * K;
+__device__ __forceinline__ double wpick(const double (&a)[K], int i, const WCtx& c) {
+    const int owner = i / K, slot = i - owner * K;
     double x = 0.0;
-    for (int k = 0; k < K; ++k)
-        if (k == slot) x = (n & 1) ? v.o[k] : v.e[k];
+    for (int k = 0; k < K; ++k) {
+        double t = a[k];
+        asm volatile("" : "+v"(t));
+        x = (k == slot) ? t : x;
+    }
     return wbc(x, owner);
 }
+
+// coefficients N-1 (even, N odd) and N-2 (odd) of v, broadcast to all lanes
+template <int K>
+__device__ __forceinline__ double wtop(const WV<K>& v, const WCtx& c) { return wpick<K>(v.e, c.Me - 1, c); }
+template <int K>
+__device__ __forceinline__ double wtop1(const WV<K>& v, const WCtx& c) { return wpick<K>(v.o, c.Mo - 1, c); }
 
 // a*x + b*y elementwise
 template <int K>
