@@ -28,7 +28,12 @@ EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_des
             "cfg_dns_set_state", "cfg_dns_get_state", "cfg_dns_reset_dt", "cfg_dns_reset_time",
             "cfg_dns_reset_gradp", "cfg_dns_reset_bulkv", "cfg_dns_time", "cfg_dns_dpdx", "cfg_dns_ubulk",
             "cfg_dns_cfl", "cfg_dns_base", "cfg_dns_device_state", "cfg_dns_profile", "cfg_dns_kernel_times",
-            "cfg_kernel_name", "cfg_nonlinear", "cfg_solve")
+            "cfg_kernel_name", "cfg_nonlinear", "cfg_solve",
+            "cfg_batch_create", "cfg_batch_destroy", "cfg_batch_add", "cfg_batch_remove", "cfg_batch_advance",
+            "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_dpdx",
+            "cfg_batch_profile", "cfg_batch_kernel_times",
+            "cfg_server_create", "cfg_server_destroy", "cfg_server_submit", "cfg_server_step", "cfg_server_result",
+            "cfg_server_memory")
 NKERNELS = 9
 
 
@@ -85,6 +90,25 @@ def library():
         "cfg_kernel_name": ([i], C.c_char_p),
         "cfg_nonlinear": ([G, F, P, P], i),
         "cfg_solve": ([G, F, d, P, P, P, P, P], i),
+        "cfg_batch_create": ([C.POINTER(v)], i),
+        "cfg_batch_destroy": ([v], i),
+        "cfg_batch_add": ([v, G, F, P, P, C.POINTER(C.c_int)], i),
+        "cfg_batch_remove": ([v, i], i),
+        "cfg_batch_advance": ([v, i], i),
+        "cfg_batch_size": ([v, C.POINTER(C.c_int), C.POINTER(C.c_int)], i),
+        "cfg_batch_get_state": ([v, i, P, P], i),
+        "cfg_batch_set_state": ([v, i, P, P], i),
+        "cfg_batch_time": ([v, i, P], i),
+        "cfg_batch_dpdx": ([v, i, P], i),
+        "cfg_batch_profile": ([v, i], i),
+        "cfg_batch_kernel_times": ([v, P, C.POINTER(C.c_longlong)], i),
+        "cfg_server_create": ([d, C.POINTER(v)], i),
+        "cfg_server_destroy": ([v], i),
+        "cfg_server_submit": ([v, G, F, P, P, i, C.POINTER(C.c_longlong)], i),
+        "cfg_server_step": ([v, C.POINTER(C.c_longlong), i, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                             C.POINTER(C.c_int)], i),
+        "cfg_server_result": ([v, C.c_longlong, P, P, P], i),
+        "cfg_server_memory": ([v, P], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -290,6 +314,137 @@ class DNS:
         W = np.zeros(self._grid.Ny)
         _check(library().cfg_dns_base(self._h, _ptr(U), _ptr(W)))
         return U, W
+
+
+class Batch:
+    """Many independent DNS integrations advanced together on one GPU.
+
+    add(fields, flags) takes the same arguments as DNS(fields, flags) and
+    returns a member id; advance(n) is DNS::advance(fields, n) for every member
+    at once. Members on the same grid share batched launches; each member's
+    result is the same as that of its own DNS.
+    """
+
+    def __init__(self):
+        self._h = C.c_void_p()
+        _check(library().cfg_batch_create(C.byref(self._h)))
+        self._shapes = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            library().cfg_batch_destroy(h)
+            self._h = None
+
+    def add(self, fields, flags):
+        u = fields[0]
+        q = fields[1] if len(fields) > 1 else None
+        g = u.grid()
+        cf = flags.c()
+        mid = C.c_int()
+        _check(library().cfg_batch_add(self._h, C.byref(g), C.byref(cf), _ptr(_f64(u.data)),
+                                       _ptr(_f64(q.data)) if q is not None else None, C.byref(mid)))
+        self._shapes[mid.value] = u.data.shape
+        return mid.value
+
+    def remove(self, member):
+        _check(library().cfg_batch_remove(self._h, int(member)))
+        self._shapes.pop(member, None)
+
+    def advance(self, nSteps=1):
+        _check(library().cfg_batch_advance(self._h, int(nSteps)))
+
+    def size(self):
+        n, k = C.c_int(), C.c_int()
+        _check(library().cfg_batch_size(self._h, C.byref(n), C.byref(k)))
+        return n.value, k.value
+
+    def state(self, member):
+        shape = self._shapes[member]
+        u = np.zeros(shape, dtype=np.complex128)
+        q = np.zeros((1,) + shape[1:], dtype=np.complex128)
+        _check(library().cfg_batch_get_state(self._h, int(member), _ptr(u.view(np.float64)),
+                                             _ptr(q.view(np.float64))))
+        return u, q
+
+    def set_state(self, member, fields):
+        u = _f64(fields[0].data)
+        q = _f64(fields[1].data) if len(fields) > 1 else None
+        _check(library().cfg_batch_set_state(self._h, int(member), _ptr(u), _ptr(q)))
+
+    def time(self, member):
+        t = C.c_double()
+        _check(library().cfg_batch_time(self._h, int(member), C.byref(t)))
+        return t.value
+
+    def dPdx(self, member):
+        v = C.c_double()
+        _check(library().cfg_batch_dpdx(self._h, int(member), C.byref(v)))
+        return v.value
+
+    def profile(self, enable=True):
+        _check(library().cfg_batch_profile(self._h, int(bool(enable))))
+
+    def kernel_times(self):
+        tot = np.zeros(NKERNELS)
+        cnt = (C.c_longlong * NKERNELS)()
+        _check(library().cfg_batch_kernel_times(self._h, _ptr(tot), cnt))
+        return {library().cfg_kernel_name(i).decode(): (float(tot[i]), int(cnt[i])) for i in range(NKERNELS)}
+
+
+class Server:
+    """Continuous batching of DNS integration requests on one GPU.
+
+    submit(fields, flags, nsteps) queues a request and returns its id; step()
+    admits queued requests while their device memory fits the budget, runs
+    one tick of every running request and returns the ids whose results have
+    reached host memory; result(id) returns (u, q, t) and releases it.
+    """
+
+    def __init__(self, budget_bytes):
+        self._h = C.c_void_p()
+        _check(library().cfg_server_create(float(budget_bytes), C.byref(self._h)))
+        self._shapes = {}
+        self._done = (C.c_longlong * 256)()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            library().cfg_server_destroy(h)
+            self._h = None
+
+    def submit(self, fields, flags, nsteps):
+        u = fields[0]
+        q = fields[1] if len(fields) > 1 else None
+        g = u.grid()
+        cf = flags.c()
+        rid = C.c_longlong()
+        _check(library().cfg_server_submit(self._h, C.byref(g), C.byref(cf), _ptr(_f64(u.data)),
+                                           _ptr(_f64(q.data)) if q is not None else None, int(nsteps),
+                                           C.byref(rid)))
+        self._shapes[rid.value] = u.data.shape
+        return rid.value
+
+    def step(self):
+        """One server step: returns (completed ids, running, queued)."""
+        nd, nr, nq = C.c_int(), C.c_int(), C.c_int()
+        _check(library().cfg_server_step(self._h, self._done, len(self._done), C.byref(nd), C.byref(nr),
+                                         C.byref(nq)))
+        return [int(self._done[i]) for i in range(nd.value)], nr.value, nq.value
+
+    def result(self, rid):
+        shape = self._shapes.pop(rid)
+        u = np.zeros(shape, dtype=np.complex128)
+        q = np.zeros((1,) + shape[1:], dtype=np.complex128)
+        t = C.c_double()
+        _check(library().cfg_server_result(self._h, C.c_longlong(rid), _ptr(u.view(np.float64)),
+                                           _ptr(q.view(np.float64)), C.byref(t)))
+        return u, q, t.value
+
+    def memory(self):
+        v = C.c_double()
+        _check(library().cfg_server_memory(self._h, C.byref(v)))
+        return v.value
 
 
 def nonlinear(u, flags):

@@ -12,8 +12,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcfgpu.so")
 
-SOURCES = ["cfg_kernels.hip", "cfg_runtime.cpp"]
-HEADERS = ["cfg_kernels.hpp", "cfg_math.hpp"]
+SOURCES = ["cfg_kernels.hip", "cfg_step.cpp", "cfg_runtime.cpp", "cfg_batch.cpp"]
+HEADERS = ["cfg_kernels.hpp", "cfg_math.hpp", "cfg_wave.hpp", "cfg_host.hpp", "cfg_step.hpp"]
 
 # Packed FP32 instructions are disabled in every HIP compile of this project.
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -33,15 +33,19 @@ def build(verbose=False):
     objs = []
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
-    for src in SOURCES:
+    procs = []
+    for src in SOURCES:  # one compiler process per source, in parallel
         obj = os.path.join(build_dir, src + ".o")
         cmd = [hipcc()] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd[1:1] = ["-x", "hip"]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise RuntimeError(f"hipcc failed on {failed}")
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:

@@ -1,0 +1,431 @@
+// Batched and served DNS integrations (include/cfgpu.h, cfg_batch_* and
+// cfg_server_*). channelflow runs its many independent integrations one
+// after another (edge-tracking bisection members, multi-shooting segments,
+// DSI Jacobian probes: cfdsi.cpp:678-775, multiShootingDSI.cpp:70-86,
+// edgetracking.cpp:472-530); here they advance together.
+//
+// Integrations on the same grid (and dealiasing) form a class. A class owns
+// one Exec (stream, work buffers, launch-table cache, memory pool) and runs
+// one tick of all its members per launch sequence; classes run on separate
+// streams and overlap on the device. Members never exchange data, and every
+// kernel treats each member on its own, so a member's result is the same alone
+// and in any mix.
+#include <chrono>
+#include <deque>
+
+#include "cfg_step.hpp"
+
+namespace cfg {
+
+struct BatchClass {
+    std::unique_ptr<Exec> ex;
+    std::vector<int> members;  // ids, in insertion order
+};
+
+struct BatchMember {
+    int id;
+    BatchClass* cls;
+    std::unique_ptr<Integrator> it;
+    long target = 0;  // steps to reach in the current advance / serving horizon
+};
+
+class Batch {
+   public:
+    ~Batch() {
+        for (auto& kv : classes_) kv.second->ex->sync();
+        members_.clear();  // integrators before their Exec
+        classes_.clear();
+    }
+
+    int add(const cfg_grid& grid, const cfg_flags& flags, const double* u, const double* q) {
+        check_flags(flags);
+        check_grid(grid);
+        const GridKey k = grid_key(grid, flags.dealiasing == CFG_DealiasXZ);
+        auto cit = classes_.find(k);
+        if (cit == classes_.end()) {
+            auto c = std::make_unique<BatchClass>();
+            c->ex = std::make_unique<Exec>(grid, flags.dealiasing == CFG_DealiasXZ);
+            cit = classes_.emplace(k, std::move(c)).first;
+        }
+        BatchClass* cls = cit->second.get();
+        auto m = std::make_unique<BatchMember>();
+        m->id = next_id_++;
+        m->cls = cls;
+        m->it = std::make_unique<Integrator>(*cls->ex, flags, u, q, nullptr, nullptr);
+        cls->members.push_back(m->id);
+        const int id = m->id;
+        members_.emplace(id, std::move(m));
+        return id;
+    }
+
+    void remove(int id) {
+        BatchMember& m = get(id);
+        auto& v = m.cls->members;
+        v.erase(std::find(v.begin(), v.end(), id));
+        members_.erase(id);  // its buffers return to the class pool (stream-ordered reuse)
+    }
+
+    BatchMember& get(int id) {
+        auto it = members_.find(id);
+        if (it == members_.end()) throw Error(CFG_ERR_ARG, "no such batch member");
+        return *it->second;
+    }
+    int size() const { return (int)members_.size(); }
+
+    // One tick of every member below its target. Returns false when none was.
+    bool tick() {
+        bool any = false;
+        for (auto& kv : classes_) {
+            BatchClass& c = *kv.second;
+            ops_.clear();
+            ptrs_.clear();
+            for (int id : c.members) {
+                BatchMember& m = *members_.at(id);
+                if (m.it->steps() < m.target) {
+                    ops_.emplace_back();
+                    m.it->next(ops_.back());
+                }
+            }
+            if (ops_.empty()) continue;
+            for (auto& o : ops_) ptrs_.push_back(&o);
+            c.ex->run(ptrs_);
+            any = true;
+        }
+        return any;
+    }
+
+    void advance(int n) {
+        for (auto& kv : members_) kv.second->target = kv.second->it->steps() + n;
+        while (tick()) {
+        }
+        sync();
+    }
+
+    void sync() {
+        for (auto& kv : classes_) kv.second->ex->sync();
+    }
+
+    void profile(bool on) {
+        for (auto& kv : classes_) {
+            kv.second->ex->kt.reset();
+            kv.second->ex->kt.on = on;
+        }
+    }
+    void kernel_times(double* ms, long long* cnt) {
+        for (int i = 0; i < K_COUNT; ++i) {
+            if (ms) ms[i] = 0.0;
+            if (cnt) cnt[i] = 0;
+        }
+        for (auto& kv : classes_) {
+            KTimer& kt = kv.second->ex->kt;
+            kt.flush();
+            for (int i = 0; i < K_COUNT; ++i) {
+                if (ms) ms[i] += kt.total_ms[i];
+                if (cnt) cnt[i] += kt.count[i];
+            }
+        }
+    }
+    int nclasses() const { return (int)classes_.size(); }
+
+   private:
+    std::map<GridKey, std::unique_ptr<BatchClass>> classes_;
+    std::map<int, std::unique_ptr<BatchMember>> members_;
+    std::deque<TickOps> ops_;
+    std::vector<TickOps*> ptrs_;
+    int next_id_ = 0;
+};
+
+// Device bytes one integration holds while admitted: its fields and
+// history, its share of the class work buffers, and its tau data (shared
+// when another member has the same grid, nu and dt; counted in full).
+inline double member_bytes(const cfg_grid& gr, const cfg_flags& f) {
+    const bool de = f.dealiasing == CFG_DealiasXZ;
+    const long kxd = de ? gr.Nx / 3 - 1 : gr.Nx / 2 - 1, kzd = de ? gr.Nz / 3 - 1 : gr.Nz / 2 - 1;
+    const double M = (double)(2 * kxd + 1) * (kzd + 1), L = gr.Ny * M;  // complex values per component
+    const double X = (double)gr.Ny * gr.Nx * (kzd + 1);
+    const double state = 16.0 * L * (4 + 4 * 4 + 3 * 4 + 3 * 3 + 3);  // cur, history, NL history, init buffers
+    const double work = 16.0 * (18 * L + 18 * X) + 8.0 * 3 * L;
+    const double tau = 8.0 * (TAU_NA * L + 6 * M) * 4;  // main + 3 init substeps
+    return state + work + tau;
+}
+
+// Continuous batching of integration requests: queued requests are admitted
+// while the device-memory budget allows, every admitted request advances one
+// tick per server step alongside the others, and a request that reaches its
+// horizon leaves at once (its slot and memory go to the next in the queue).
+// Results come back through pinned host memory without stalling the others.
+class Server {
+   public:
+    explicit Server(double budget) : budget_(budget) {}
+    ~Server() {
+        batch_.sync();
+        for (auto& kv : done_)
+            if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+        for (auto& kv : done_)
+            if (kv.second.host) (void)hipHostFree(kv.second.host);
+    }
+
+    long long submit(const cfg_grid& g, const cfg_flags& f, const double* u, const double* q, int nsteps) {
+        check_flags(f);
+        check_grid(g);
+        if (nsteps < 0) throw Error(CFG_ERR_ARG, "negative horizon");
+        const double bytes = member_bytes(g, f);
+        if (bytes > budget_) throw Error(CFG_ERR_ARG, "request does not fit the device-memory budget");
+        Req r;
+        r.id = next_id_++;
+        r.grid = g;
+        r.flags = f;
+        r.nsteps = nsteps;
+        r.bytes = bytes;
+        const size_t nu = (size_t)2 * 3 * g.Ny * g.Nx * (g.Nz / 2 + 1), nq = nu / 3;
+        r.u.assign(u, u + nu);
+        if (q) r.q.assign(q, q + nq);
+        queue_.push_back(std::move(r));
+        return queue_.back().id;
+    }
+
+    // One server step: admit, one tick of every running request, retire the
+    // ones at their horizon, report results that have reached the host.
+    void step(long long* done_ids, int max_done, int* ndone, int* nrun, int* nqueued) {
+        admit();
+        batch_.tick();
+        for (auto it = running_.begin(); it != running_.end();) {
+            BatchMember& m = batch_.get(it->second.member);
+            if (m.it->steps() >= m.target) {
+                retire(it->first, it->second);
+                it = running_.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        int n = 0;
+        for (auto& kv : done_) {
+            Done& d = kv.second;
+            if (d.reported || !d.ev) continue;
+            if (hipEventQuery(d.ev) != hipSuccess) continue;
+            d.ready = true;
+            if (n < max_done) {
+                if (done_ids) done_ids[n] = kv.first;
+                d.reported = true;
+                ++n;
+            }
+        }
+        if (ndone) *ndone = n;
+        if (nrun) *nrun = (int)running_.size();
+        if (nqueued) *nqueued = (int)queue_.size();
+    }
+
+    void result(long long id, double* u_spec, double* q_spec, double* t) {
+        auto it = done_.find(id);
+        if (it == done_.end()) throw Error(CFG_ERR_ARG, "no such completed request");
+        Done& d = it->second;
+        HIPCHK(hipEventSynchronize(d.ev));
+        std::vector<double> hu(d.host, d.host + d.nu), hq(d.host + d.nu, d.host + d.nu + d.nq);
+        if (u_spec) unpack(d.g, 3, hu, u_spec);
+        if (q_spec) unpack(d.g, 1, hq, q_spec);
+        if (t) *t = d.t;
+        (void)hipEventDestroy(d.ev);
+        (void)hipHostFree(d.host);
+        done_.erase(it);
+    }
+
+    double used() const { return used_; }
+
+   private:
+    struct Req {
+        long long id;
+        cfg_grid grid;
+        cfg_flags flags;
+        int nsteps;
+        double bytes;
+        std::vector<double> u, q;
+    };
+    struct Run {
+        int member;
+        double bytes;
+    };
+    struct Done {
+        Geom g;
+        double* host = nullptr;
+        size_t nu = 0, nq = 0;
+        hipEvent_t ev = nullptr;
+        double t = 0;
+        bool ready = false, reported = false;
+    };
+
+    void admit() {
+        while (!queue_.empty() && used_ + queue_.front().bytes <= budget_) {
+            Req r = std::move(queue_.front());
+            queue_.pop_front();
+            const int mid = batch_.add(r.grid, r.flags, r.u.data(), r.q.empty() ? nullptr : r.q.data());
+            batch_.get(mid).target = r.nsteps;
+            running_.emplace(r.id, Run{mid, r.bytes});
+            used_ += r.bytes;
+        }
+    }
+
+    void retire(long long id, const Run& run) {
+        BatchMember& m = batch_.get(run.member);
+        Exec& ex = *m.cls->ex;
+        Done d;
+        d.g = ex.geo->g;
+        d.nu = ex.nfield;
+        d.nq = ex.nscal;
+        d.t = m.it->time();
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&d.host), (d.nu + d.nq) * sizeof(double)));
+        const StateRef s = m.it->state();
+        HIPCHK(hipMemcpyAsync(d.host, s.u, d.nu * sizeof(double), hipMemcpyDeviceToHost, ex.st));
+        HIPCHK(hipMemcpyAsync(d.host + d.nu, s.q, d.nq * sizeof(double), hipMemcpyDeviceToHost, ex.st));
+        HIPCHK(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(d.ev, ex.st));
+        batch_.remove(run.member);  // buffers go back to the class pool, reused only by later work
+        used_ -= run.bytes;
+        done_.emplace(id, d);
+    }
+
+    Batch batch_;
+    double budget_, used_ = 0;
+    std::deque<Req> queue_;
+    std::map<long long, Run> running_;
+    std::map<long long, Done> done_;
+    long long next_id_ = 0;
+};
+
+}  // namespace cfg
+
+using namespace cfg;
+
+struct cfg_batch {
+    Batch b;
+};
+struct cfg_server {
+    explicit cfg_server(double budget) : s(budget) {}
+    Server s;
+};
+
+extern "C" {
+
+int cfg_batch_create(cfg_batch** out) {
+    return guarded([&] {
+        if (!out) throw Error(CFG_ERR_ARG, "null argument");
+        *out = new cfg_batch();
+    });
+}
+
+int cfg_batch_destroy(cfg_batch* b) {
+    return guarded([&] { delete b; });
+}
+
+int cfg_batch_add(cfg_batch* b, const cfg_grid* grid, const cfg_flags* flags, const double* u_spec,
+                  const double* q_spec, int* member) {
+    return guarded([&] {
+        if (!b || !grid || !flags || !u_spec || !member) throw Error(CFG_ERR_ARG, "null argument");
+        *member = b->b.add(*grid, *flags, u_spec, q_spec);
+    });
+}
+
+int cfg_batch_remove(cfg_batch* b, int member) {
+    return guarded([&] {
+        if (!b) throw Error(CFG_ERR_ARG, "null batch");
+        b->b.remove(member);
+    });
+}
+
+int cfg_batch_advance(cfg_batch* b, int nsteps) {
+    return guarded([&] {
+        if (!b || nsteps < 0) throw Error(CFG_ERR_ARG, "bad argument");
+        b->b.advance(nsteps);
+    });
+}
+
+int cfg_batch_size(const cfg_batch* b, int* n, int* nclasses) {
+    return guarded([&] {
+        if (!b) throw Error(CFG_ERR_ARG, "null batch");
+        if (n) *n = b->b.size();
+        if (nclasses) *nclasses = b->b.nclasses();
+    });
+}
+
+int cfg_batch_get_state(cfg_batch* b, int member, double* u_spec, double* q_spec) {
+    return guarded([&] {
+        if (!b) throw Error(CFG_ERR_ARG, "null batch");
+        b->b.get(member).it->get_state(u_spec, q_spec);
+    });
+}
+
+int cfg_batch_set_state(cfg_batch* b, int member, const double* u_spec, const double* q_spec) {
+    return guarded([&] {
+        if (!b) throw Error(CFG_ERR_ARG, "null batch");
+        b->b.get(member).it->set_state(u_spec, q_spec);
+    });
+}
+
+int cfg_batch_time(cfg_batch* b, int member, double* t) {
+    return guarded([&] {
+        if (!b || !t) throw Error(CFG_ERR_ARG, "null argument");
+        *t = b->b.get(member).it->time();
+    });
+}
+
+int cfg_batch_dpdx(cfg_batch* b, int member, double* v) {
+    return guarded([&] {
+        if (!b || !v) throw Error(CFG_ERR_ARG, "null argument");
+        *v = b->b.get(member).it->dPdx();
+    });
+}
+
+int cfg_batch_profile(cfg_batch* b, int enable) {
+    return guarded([&] {
+        if (!b) throw Error(CFG_ERR_ARG, "null batch");
+        b->b.profile(enable != 0);
+    });
+}
+
+int cfg_batch_kernel_times(cfg_batch* b, double* total_ms, long long* counts) {
+    return guarded([&] {
+        if (!b) throw Error(CFG_ERR_ARG, "null batch");
+        b->b.kernel_times(total_ms, counts);
+    });
+}
+
+int cfg_server_create(double budget_bytes, cfg_server** out) {
+    return guarded([&] {
+        if (!out || !(budget_bytes > 0)) throw Error(CFG_ERR_ARG, "bad argument");
+        *out = new cfg_server(budget_bytes);
+    });
+}
+
+int cfg_server_destroy(cfg_server* s) {
+    return guarded([&] { delete s; });
+}
+
+int cfg_server_submit(cfg_server* s, const cfg_grid* grid, const cfg_flags* flags, const double* u_spec,
+                      const double* q_spec, int nsteps, long long* request) {
+    return guarded([&] {
+        if (!s || !grid || !flags || !u_spec || !request) throw Error(CFG_ERR_ARG, "null argument");
+        *request = s->s.submit(*grid, *flags, u_spec, q_spec, nsteps);
+    });
+}
+
+int cfg_server_step(cfg_server* s, long long* done, int max_done, int* ndone, int* nrunning, int* nqueued) {
+    return guarded([&] {
+        if (!s || max_done < 0) throw Error(CFG_ERR_ARG, "bad argument");
+        s->s.step(done, max_done, ndone, nrunning, nqueued);
+    });
+}
+
+int cfg_server_result(cfg_server* s, long long request, double* u_spec, double* q_spec, double* t) {
+    return guarded([&] {
+        if (!s) throw Error(CFG_ERR_ARG, "null server");
+        s->s.result(request, u_spec, q_spec, t);
+    });
+}
+
+int cfg_server_memory(const cfg_server* s, double* used_bytes) {
+    return guarded([&] {
+        if (!s || !used_bytes) throw Error(CFG_ERR_ARG, "null argument");
+        *used_bytes = s->s.used();
+    });
+}
+
+}  // extern "C"

@@ -267,14 +267,22 @@ __device__ __forceinline__ void tau_P_and_v_wave(const WCtx& c, const double* cm
     }
 }
 
-// Block of TAU_WPB consecutive modes: the right-hand side rows [c][n][m0..m0+WPB)
-// are read with full-line loads into LDS as [mode][c][n], each wave solves its
-// mode in registers, and u, v, w, q leave through LDS the same way.
+// Block of TAU_WPB consecutive modes of member blockIdx.y: the right-hand side
+// rows [c][n][m0..m0+WPB) are formed from the member's terms (the multistep /
+// Runge-Kutta combination, dnsalgo.cpp:218-224, 668-673) with full-line loads
+// into LDS as [mode][c][n], each wave solves its mode in registers, and u, v,
+// w, q leave through LDS the same way.
 template <int K>
-__global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, TauSet s, double nu, int taucorr,
-                                                                 SolveConst zc, const double2* __restrict__ rhs,
-                                                                 double2* __restrict__ u, double2* __restrict__ q) {
+__global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const TauMember* __restrict__ tab) {
     extern __shared__ double2 lds[];
+    const TauMember& me = tab[blockIdx.y];
+    const TauSet s = me.set;
+    const double nu = me.nu;
+    const int taucorr = me.taucorr;
+    const SolveConst zc = me.sc;
+    double2* __restrict__ u = reinterpret_cast<double2*>(me.u);
+    double2* __restrict__ q = reinterpret_cast<double2*>(me.q);
+    const int nterms = me.nterms;
     const int Ny = g.Ny;
     const int nthr = 64 * TAU_WPB;
     const int m0 = blockIdx.x * TAU_WPB;
@@ -284,7 +292,20 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, TauSet 
     for (int idx = threadIdx.x; idx < 3 * Ny * TAU_WPB; idx += nthr) {
         const int ww = idx % TAU_WPB, rest = idx / TAU_WPB, n = rest % Ny, comp = rest / Ny;
         const int mm = m0 + ww;
-        lds[(ww * 3 + comp) * Ny + n] = (mm < g.M) ? rhs[((long)comp * Ny + n) * g.M + mm] : make_double2(0.0, 0.0);
+        double2 acc = make_double2(0.0, 0.0);
+        if (mm < g.M) {
+            const long off = ((long)comp * Ny + n) * g.M + mm;
+#pragma unroll
+            for (int j = 0; j < MAX_TERMS; ++j) {
+                if (j < nterms) {
+                    const double2 v = reinterpret_cast<const double2*>(me.in[j])[off];
+                    const double c = me.coef[j];
+                    acc.x += c * v.x;
+                    acc.y += c * v.y;
+                }
+            }
+        }
+        lds[(ww * 3 + comp) * Ny + n] = acc;
     }
     __syncthreads();
     WCtx c;
@@ -401,21 +422,16 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, TauSet 
     }
 }
 
-void launch_tau_solve(const Geom& g, const TauSet& s, double nu, bool taucorr, const SolveConst& c, const double* rhs,
-                      double* u, double* q, double* scratch, hipStream_t st) {
-    const int blocks = (g.M + TAU_WPB - 1) / TAU_WPB;
+void launch_tau_solve(const Geom& g, int n, const TauMember* tab, hipStream_t st) {
+    const dim3 grid((g.M + TAU_WPB - 1) / TAU_WPB, n);
     const size_t lds = (size_t)3 * g.Ny * TAU_WPB * sizeof(double2);
-    const double2* r2 = reinterpret_cast<const double2*>(rhs);
-    double2* u2 = reinterpret_cast<double2*>(u);
-    double2* q2 = reinterpret_cast<double2*>(q);
     const int Me = (g.Ny + 1) / 2;
-    const int tc = taucorr ? 1 : 0;
     if (Me <= 64)
-        k_tau_solve_wave<1><<<blocks, 64 * TAU_WPB, lds, st>>>(g, s, nu, tc, c, r2, u2, q2);
+        k_tau_solve_wave<1><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
     else if (Me <= 128)
-        k_tau_solve_wave<2><<<blocks, 64 * TAU_WPB, lds, st>>>(g, s, nu, tc, c, r2, u2, q2);
+        k_tau_solve_wave<2><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
     else
-        k_tau_solve_wave<3><<<blocks, 64 * TAU_WPB, lds, st>>>(g, s, nu, tc, c, r2, u2, q2);
+        k_tau_solve_wave<3><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
 }
 
 // ===========================================================================
@@ -447,8 +463,14 @@ void launch_combine(long n, int nterms, const double* coef, const double* const*
 // ===========================================================================
 // NSE::linear on live modes (nse.cpp:393-477), one thread per mode
 // ===========================================================================
-__global__ void __launch_bounds__(64) k_linear(Geom g, double nu, LinearConst c, const double* u, const double* q,
-                                               double* out, double* scratch) {
+__global__ void __launch_bounds__(64) k_linear(Geom g, const LinearMember* __restrict__ tab) {
+    const LinearMember& me = tab[blockIdx.y];
+    const double nu = me.nu;
+    const LinearConst c = me.c;
+    const double* u = me.u;
+    const double* q = me.q;
+    double* out = me.out;
+    double* scratch = me.scratch;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= g.M) return;
     const int ikx = m % g.nkx, ikz = m / g.nkx;
@@ -510,10 +532,9 @@ __global__ void __launch_bounds__(64) k_linear(Geom g, double nu, LinearConst c,
     }
 }
 
-void launch_linear(const Geom& g, double nu, const LinearConst& c, const double* u, const double* q, double* out,
-                   double* scratch, hipStream_t st) {
+void launch_linear(const Geom& g, int n, const LinearMember* tab, hipStream_t st) {
     const int blk = 64;
-    k_linear<<<(g.M + blk - 1) / blk, blk, 0, st>>>(g, nu, c, u, q, out, scratch);
+    k_linear<<<dim3((g.M + blk - 1) / blk, n), blk, 0, st>>>(g, tab);
 }
 
 // ===========================================================================
@@ -524,8 +545,13 @@ constexpr int YT = 8;  // y outputs per thread in the y-transforms
 // Chebyshev -> y values. Outputs per method:
 //   ROT : u v w wx wy wz      CONV/SKEW : u v w du/dy dv/dy dw/dy      DIV : u v w
 template <int METHOD>
-__global__ void __launch_bounds__(64) k_yinv(Geom g, YMats ym, const double2* __restrict__ u, double2* __restrict__ yp,
-                                             double Vsuck) {
+__global__ void __launch_bounds__(64) k_yinv(Geom g, YMats ym, const NLMember* __restrict__ tab, double2* __restrict__ yp0,
+                                             long sy) {
+    const NLMember& me = tab[blockIdx.z];
+    const double2* __restrict__ u = reinterpret_cast<const double2*>(me.u);
+    double2* __restrict__ yp = yp0 + blockIdx.z * sy;
+    const double* __restrict__ base = me.base;
+    const double Vsuck = me.Vsuck;
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const int y0 = blockIdx.y * YT;
     if (m >= g.M) return;
@@ -560,10 +586,10 @@ __global__ void __launch_bounds__(64) k_yinv(Geom g, YMats ym, const double2* __
         const int y = y0 + j;
         if (y >= Ny) break;
         if (m == 0) {  // base flow on the kx = kz = 0 mode (nse.cpp:28-36)
-            P[0][j].x += ym.base[0 * Ny + y];
-            Q[0][j].x += ym.base[1 * Ny + y];
-            P[2][j].x += ym.base[2 * Ny + y];
-            Q[2][j].x += ym.base[3 * Ny + y];
+            P[0][j].x += base[0 * Ny + y];
+            Q[0][j].x += base[1 * Ny + y];
+            P[2][j].x += base[2 * Ny + y];
+            Q[2][j].x += base[3 * Ny + y];
             P[1][j].x -= Vsuck;
         }
         double2 o[6];
@@ -589,9 +615,11 @@ __global__ void __launch_bounds__(64) k_yinv(Geom g, YMats ym, const double2* __
 // inverse x-FFT: yp [F][Ny][nkz][nkx] -> xp [Fo][Ny][Nx][nkz]; output f >= nsrc is
 // d/dx of source field f - nsrc. XL kz-lines per block.
 template <int NX>
-__global__ void __launch_bounds__(256) k_xinv(Geom g, FFTPlan px, int XL, int nsrc, const double2* __restrict__ yp,
-                                              double2* __restrict__ xp) {
+__global__ void __launch_bounds__(256) k_xinv(Geom g, FFTPlan px, int XL, int nsrc, const double2* __restrict__ yp0,
+                                              long sy, double2* __restrict__ xp0, long sx) {
     extern __shared__ double2 lds[];
+    const double2* __restrict__ yp = yp0 + blockIdx.z * sy;
+    double2* __restrict__ xp = xp0 + blockIdx.z * sx;
     const int Nx = g.Nx, nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
     const int nchunk = (nkz + XL - 1) / XL;
     const int y = blockIdx.x / nchunk, ikz0 = (blockIdx.x % nchunk) * XL;
@@ -627,9 +655,11 @@ __global__ void __launch_bounds__(256) k_xinv(Geom g, FFTPlan px, int XL, int ns
 
 // forward x-FFT: zo [G][Ny][Nx][nkz] -> fo [G][Ny][nkz][nkx], scaled by 1/(Nx Nz)
 template <int NX>
-__global__ void __launch_bounds__(256) k_xfwd(Geom g, FFTPlan px, int XL, const double2* __restrict__ zo,
-                                              double2* __restrict__ fo) {
+__global__ void __launch_bounds__(256) k_xfwd(Geom g, FFTPlan px, int XL, const double2* __restrict__ zo0, long sx,
+                                              double2* __restrict__ fo0, long sy) {
     extern __shared__ double2 lds[];
+    const double2* __restrict__ zo = zo0 + blockIdx.z * sx;
+    double2* __restrict__ fo = fo0 + blockIdx.z * sy;
     const int Nx = g.Nx, nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
     const int nchunk = (nkz + XL - 1) / XL;
     const int y = blockIdx.x / nchunk, ikz0 = (blockIdx.x % nchunk) * XL;
@@ -739,9 +769,13 @@ __device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int g
 // u.grad u / omega x u part and the rotation term) then uu00 uu01 uu02 uu11
 // uu12 uu22 for DIV/SKEW.
 template <int METHOD, int NZ>
-__global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, double rotation, int want_g,
-                                               const double2* __restrict__ xp, double2* __restrict__ zo) {
+__global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, const NLMember* __restrict__ tab,
+                                               int want_g, const double2* __restrict__ xp0, double2* __restrict__ zo0,
+                                               long sx) {
     extern __shared__ double2 lds[];
+    const double rotation = tab[blockIdx.y].rotation;
+    const double2* __restrict__ xp = xp0 + blockIdx.y * sx;
+    double2* __restrict__ zo = zo0 + blockIdx.y * sx;
     const int nxc = (g.Nx + XB - 1) / XB;
     ZCtx c;
     c.g = g;
@@ -863,8 +897,10 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, doubl
 //   SKEW     : G_i = g_i + 0.5 (ikx uu_i0 + ikz uu_i2),        H_i = 0.5 uu_i1
 //   DIV      : G_i = [g_i] + (ikx uu_i0 + ikz uu_i2),          H_i = uu_i1
 template <int METHOD>
-__global__ void __launch_bounds__(64) k_yfwd(Geom g, YMats ym, int have_g, const double2* __restrict__ fo,
-                                             double2* __restrict__ f) {
+__global__ void __launch_bounds__(64) k_yfwd(Geom g, YMats ym, int have_g, const double2* __restrict__ fo0, long sy,
+                                             const NLMember* __restrict__ tab) {
+    const double2* __restrict__ fo = fo0 + blockIdx.z * sy;
+    double2* __restrict__ f = reinterpret_cast<double2*>(tab[blockIdx.z].f);
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     const int k0 = blockIdx.y * YT;
     if (m >= g.M) return;
@@ -945,113 +981,116 @@ static void allow_lds(K kernel, size_t bytes) {
 #define CFG_FFT_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 
 template <int NX>
-static void launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, const double2* in,
-                     double2* out, hipStream_t st) {
+static void launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, int n, const double2* in,
+                     long sin, double2* out, long sout, hipStream_t st) {
     const int XL = xfft_lines(g);
     const int nchunk = (g.nkz + XL - 1) / XL;
     const size_t lds = (size_t)(g.Nx + 2 * XL * g.Nx) * sizeof(double2);
+    const dim3 grid(g.Ny * nchunk, nf, n);
     if (inverse) {
         allow_lds(k_xinv<NX>, lds);
-        k_xinv<NX><<<dim3(g.Ny * nchunk, nf), 256, lds, st>>>(g, px, XL, nsrc, in, out);
+        k_xinv<NX><<<grid, 256, lds, st>>>(g, px, XL, nsrc, in, sin, out, sout);
     } else {
         allow_lds(k_xfwd<NX>, lds);
-        k_xfwd<NX><<<dim3(g.Ny * nchunk, nf), 256, lds, st>>>(g, px, XL, in, out);
+        k_xfwd<NX><<<grid, 256, lds, st>>>(g, px, XL, in, sin, out, sout);
     }
 }
 
-static void launch_xfft(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, const double2* in,
-                        double2* out, hipStream_t st) {
+// in/out strides are per member, in double2
+static void launch_xfft(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, int n, const double2* in,
+                        long sin, double2* out, long sout, hipStream_t st) {
     switch (g.Nx) {
 #define CASE_X(N) \
-    case N: launch_x<N>(inverse, g, px, nsrc, nf, in, out, st); return;
+    case N: launch_x<N>(inverse, g, px, nsrc, nf, n, in, sin, out, sout, st); return;
         CFG_FFT_SIZES(CASE_X)
 #undef CASE_X
-        default: launch_x<0>(inverse, g, px, nsrc, nf, in, out, st);
+        default: launch_x<0>(inverse, g, px, nsrc, nf, n, in, sin, out, sout, st);
     }
 }
 
 template <int METHOD, int NZ>
-static void launch_z_t(const Geom& g, const FFTPlan& pz, double rotation, int want_g, const double2* xp,
-                       double2* zo, hipStream_t st) {
+static void launch_z_t(const Geom& g, const FFTPlan& pz, int n, const NLMember* tab, int want_g, const double2* xp,
+                       double2* zo, long sx, hipStream_t st) {
     const int nphys = (METHOD == NL_ROT) ? 6 : 9;
     const int XB = zpass_xb(g, nphys);
     const size_t lds = (size_t)g.Nz * sizeof(double2) + (size_t)XB * g.Nz * (2 * sizeof(double2) + nphys * sizeof(double));
-    const int grid = g.Ny * ((g.Nx + XB - 1) / XB);
+    const dim3 grid(g.Ny * ((g.Nx + XB - 1) / XB), n);
     allow_lds(k_zpass<METHOD, NZ>, lds);
-    k_zpass<METHOD, NZ><<<grid, 256, lds, st>>>(g, pz, XB, rotation, want_g, xp, zo);
+    k_zpass<METHOD, NZ><<<grid, 256, lds, st>>>(g, pz, XB, tab, want_g, xp, zo, sx);
 }
 
 template <int METHOD>
-static void launch_z(const Geom& g, const FFTPlan& pz, double rotation, int want_g, const double2* xp, double2* zo,
-                     hipStream_t st) {
+static void launch_z(const Geom& g, const FFTPlan& pz, int n, const NLMember* tab, int want_g, const double2* xp,
+                     double2* zo, long sx, hipStream_t st) {
     switch (g.Nz) {
 #define CASE_Z(N) \
-    case N: launch_z_t<METHOD, N>(g, pz, rotation, want_g, xp, zo, st); return;
+    case N: launch_z_t<METHOD, N>(g, pz, n, tab, want_g, xp, zo, sx, st); return;
         CFG_FFT_SIZES(CASE_Z)
 #undef CASE_Z
-        default: launch_z_t<METHOD, 0>(g, pz, rotation, want_g, xp, zo, st);
+        default: launch_z_t<METHOD, 0>(g, pz, n, tab, want_g, xp, zo, sx, st);
     }
 }
 
-void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, double Vsuck,
-                      double rotation, const double* u, double* f, const NLWork& w, hipStream_t st, KTimer* kt) {
+// Nonlinear term of n members of one grid class. rotating: some member has a
+// nonzero rotation (only then does the divergence form compute g).
+void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, int n,
+                      const NLMember* tab, bool rotating, const NLWork& w, hipStream_t st, KTimer* kt) {
     KTimer dummy;
     KTimer& T = kt ? *kt : dummy;
-    const dim3 ygrid((g.M + 63) / 64, (g.Ny + YT - 1) / YT);
-    const double2* u2 = reinterpret_cast<const double2*>(u);
+    const dim3 ygrid((g.M + 63) / 64, (g.Ny + YT - 1) / YT, n);
+    const long sy = nlwork_y_doubles(g) / 2, sx = nlwork_x_doubles(g) / 2;  // per-member strides in double2
     double2* yp = reinterpret_cast<double2*>(w.yp);
     double2* xp = reinterpret_cast<double2*>(w.xp);
     double2* zo = reinterpret_cast<double2*>(w.zo);
     double2* fo = reinterpret_cast<double2*>(w.fo);
-    double2* f2 = reinterpret_cast<double2*>(f);
     int nsrc = 3, nxout = 3;
     hipEvent_t ev = T.begin();
     switch (method) {
         case NL_ROT:
-            k_yinv<NL_ROT><<<ygrid, 64, 0, st>>>(g, ym, u2, yp, Vsuck);
+            k_yinv<NL_ROT><<<ygrid, 64, 0, st>>>(g, ym, tab, yp, sy);
             nsrc = 6; nxout = 6;
             break;
         case NL_CONV:
-            k_yinv<NL_CONV><<<ygrid, 64, 0, st>>>(g, ym, u2, yp, Vsuck);
+            k_yinv<NL_CONV><<<ygrid, 64, 0, st>>>(g, ym, tab, yp, sy);
             nsrc = 6; nxout = 9;
             break;
         case NL_SKEW:
-            k_yinv<NL_SKEW><<<ygrid, 64, 0, st>>>(g, ym, u2, yp, Vsuck);
+            k_yinv<NL_SKEW><<<ygrid, 64, 0, st>>>(g, ym, tab, yp, sy);
             nsrc = 6; nxout = 9;
             break;
         default:
-            k_yinv<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, u2, yp, Vsuck);
+            k_yinv<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, tab, yp, sy);
             nsrc = 3; nxout = 3;
     }
     T.end(K_YINV, ev);
     ev = T.begin();
     // d/dx outputs (fields nsrc..) derive from fields 0..2 only
-    launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, yp, xp, st);
+    launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, n, yp, sy, xp, sx, st);
     T.end(K_XINV, ev);
-    const int want_g = (method != NL_DIV) || (rotation != 0.0);
+    const int want_g = (method != NL_DIV) || rotating;
     ev = T.begin();
     switch (method) {
-        case NL_ROT: launch_z<NL_ROT>(g, pz, rotation, want_g, xp, zo, st); break;
-        case NL_CONV: launch_z<NL_CONV>(g, pz, rotation, want_g, xp, zo, st); break;
-        case NL_SKEW: launch_z<NL_SKEW>(g, pz, rotation, want_g, xp, zo, st); break;
-        default: launch_z<NL_DIV>(g, pz, rotation, want_g, xp, zo, st);
+        case NL_ROT: launch_z<NL_ROT>(g, pz, n, tab, want_g, xp, zo, sx, st); break;
+        case NL_CONV: launch_z<NL_CONV>(g, pz, n, tab, want_g, xp, zo, sx, st); break;
+        case NL_SKEW: launch_z<NL_SKEW>(g, pz, n, tab, want_g, xp, zo, sx, st); break;
+        default: launch_z<NL_DIV>(g, pz, n, tab, want_g, xp, zo, sx, st);
     }
     T.end(K_ZPASS, ev);
     ev = T.begin();
     // forward x-FFT of the z-pass outputs that exist
-    if (want_g) launch_xfft(false, g, px, 0, 3, zo, fo, st);
+    if (want_g) launch_xfft(false, g, px, 0, 3, n, zo, sx, fo, sy, st);
     if (method == NL_DIV || method == NL_SKEW) {
         const long off = 3L * g.Ny * g.Nx * g.nkz;
         const long offo = 3L * g.Ny * g.M;
-        launch_xfft(false, g, px, 0, 6, zo + off, fo + offo, st);
+        launch_xfft(false, g, px, 0, 6, n, zo + off, sx, fo + offo, sy, st);
     }
     T.end(K_XFWD, ev);
     ev = T.begin();
     switch (method) {
-        case NL_ROT: k_yfwd<NL_ROT><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, f2); break;
-        case NL_CONV: k_yfwd<NL_CONV><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, f2); break;
-        case NL_SKEW: k_yfwd<NL_SKEW><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, f2); break;
-        default: k_yfwd<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, want_g, fo, f2);
+        case NL_ROT: k_yfwd<NL_ROT><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, sy, tab); break;
+        case NL_CONV: k_yfwd<NL_CONV><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, sy, tab); break;
+        case NL_SKEW: k_yfwd<NL_SKEW><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, sy, tab); break;
+        default: k_yfwd<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, want_g, fo, sy, tab);
     }
     T.end(K_YFWD, ev);
 }

@@ -99,7 +99,6 @@ struct YMats {            // y-direction operators, row-major [row][col], Ny x N
     const double* TD;     // T * (d/dy)
     const double* Ti;     // values -> coefficients
     const double* DTi;    // (d/dy) * Ti
-    const double* base;   // [4][Ny]: U(y), dU/dy(y), W(y), dW/dy(y) at the y_n
 };
 
 // Per-mode tau data for one lambda_t, mode-major: array k (TAU_NA of them:
@@ -119,6 +118,7 @@ struct SolveConst {  // kx=kz=0 mode data (nse.cpp:505-547)
     double dPdxRef, dPdzRef;
     double umean, wmean;  // UbulkRef - UbulkBase, WbulkRef - WbulkBase
     int bulk;             // 1: BulkVelocity constraint
+    int reserved;         // 0 (launch tables are compared byte-wise: no padding)
     double* dPdx_out;     // device scalars written by the bulk solve
     double* dPdz_out;
 };
@@ -128,33 +128,74 @@ struct LinearConst {  // kx=kz=0 constants of NSE::linear (nse.cpp:437-472)
     const double* nuWbaseyy;
     double dPdxRef, dPdzRef;
     int bulk;
+    int reserved;  // 0
     double nuUbase_slope, nuWbase_slope;  // nu (U'(b) - U'(a)) / Ly etc.
+};
+
+// ---- per-member launch tables ----------------------------------------------
+// Every kernel of the time step runs over n members of one grid class at once
+// (blockIdx.y or blockIdx.z = member). Pointers and parameters that differ
+// between members come from a device-resident table of these structs.
+
+constexpr int MAX_TERMS = 8;
+
+// Tau solve of one member: rhs = sum_j coef[j] in[j] (3 components, compact
+// layout), solved into u (3 components) and q (1 component).
+struct TauMember {
+    const double* in[MAX_TERMS];
+    double coef[MAX_TERMS];
+    int nterms;
+    int taucorr;
+    double nu;
+    TauSet set;
+    SolveConst sc;
+    double* u;
+    double* q;
+};
+
+// NSE::linear of one member: out = nu (u'' - kappa^2 u) - grad q (+ constants).
+// scratch holds 3 real Ny x M arrays.
+struct LinearMember {
+    const double* u;
+    const double* q;
+    double* out;
+    double* scratch;
+    double nu;
+    LinearConst c;
+};
+
+// Nonlinear term of one member: f = N(u + base flow).
+struct NLMember {
+    const double* u;     // compact velocity [3][Ny][M] complex
+    double* f;           // compact nonlinear term [3][Ny][M] complex
+    const double* base;  // [4][Ny]: U(y), dU/dy(y), W(y), dW/dy(y) at the y_n
+    double Vsuck, rotation;
 };
 
 // ---- tau ------------------------------------------------------------------
 void launch_tau_setup(const Geom& g, const TauSet& s, double nu, bool taucorr, double* scratch, hipStream_t st);
-void launch_tau_solve(const Geom& g, const TauSet& s, double nu, bool taucorr, const SolveConst& c,
-                      const double* rhs, double* u, double* q, double* scratch, hipStream_t st);
+void launch_tau_solve(const Geom& g, int n, const TauMember* tab, hipStream_t st);
 
 // ---- elementwise ----------------------------------------------------------
 // out = sum_j coef[j] * in[j], over n doubles (up to 8 terms)
 void launch_combine(long n, int nterms, const double* coef, const double* const* in, double* out, hipStream_t st);
-// NSE::linear on live modes: out = nu (u'' - kappa^2 u) - grad q (+ constants)
-void launch_linear(const Geom& g, double nu, const LinearConst& c, const double* u, const double* q, double* out,
-                   double* scratch, hipStream_t st);
+void launch_linear(const Geom& g, int n, const LinearMember* tab, hipStream_t st);
 
 // ---- nonlinear term ---------------------------------------------------------
 // LDS bytes a block may use (the device's opt-in limit); default 64 KiB.
 void set_lds_budget(size_t bytes);
 
+// Work buffers of member 0; member p uses base + p * (per-member size below).
 struct NLWork {
     double* yp;  // [6][Ny][M] complex
     double* xp;  // [9][Ny][Nx][nkz] complex
     double* zo;  // [9][Ny][Nx][nkz] complex
     double* fo;  // [9][Ny][M] complex
 };
-void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method,
-                      double Vsuck, double rotation, const double* u, double* f, const NLWork& w, hipStream_t st,
-                      KTimer* kt = nullptr);
+inline long nlwork_y_doubles(const Geom& g) { return 2L * 9 * g.Ny * g.M; }         // yp and fo
+inline long nlwork_x_doubles(const Geom& g) { return 2L * 9 * g.Ny * g.Nx * g.nkz; }  // xp and zo
+
+void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, int n,
+                      const NLMember* tab, bool rotating, const NLWork& w, hipStream_t st, KTimer* kt = nullptr);
 
 }  // namespace cfg

@@ -1,0 +1,172 @@
+// Time stepping of channelflow's DNS (dns.cpp, dnsalgo.cpp, nse.cpp) as a
+// sequence of "ticks". A tick is one nonlinear-term evaluation plus what
+// follows it before the next one: device copies, the Runge-Kutta state
+// update, NSE::linear and the tau solve. Every algorithm step is a fixed
+// number of ticks (SBDF: 1, SMRK2/CNRK2: 3, an initialisation step: 1 push
+// + the init algorithm's ticks).
+//
+// An Integrator holds the state of one integration and produces its ticks; an
+// Exec runs the ticks of any number of integrations on one grid as batched
+// launches on its stream. A single DNS is an Exec with one Integrator; a batch
+// is an Exec per grid class with many.
+#pragma once
+
+#include "cfg_host.hpp"
+
+namespace cfg {
+
+// Work of one integration for one tick, in execution order.
+struct TickOps {
+    struct Copy {
+        double* dst;
+        const double* src;
+        size_t n;  // doubles
+    };
+    int ncopy = 0;
+    Copy copy[4];
+    bool nlon = false;    // a nonlinear-term evaluation (every time-step tick has one)
+    int method = NL_ROT;  // its form
+    NLMember nl{};
+    bool comb = false;  // out = sum_j ccoef[j] cin[j] over one 3-component field
+    int cn = 0;
+    double ccoef[MAX_TERMS];
+    const double* cin[MAX_TERMS];
+    double* cout = nullptr;
+    bool lin = false;
+    LinearMember linm{};
+    bool solve = false;
+    TauMember tau{};
+
+    void add_copy(double* dst, const double* src, size_t n) { copy[ncopy++] = Copy{dst, src, n}; }
+};
+
+class Exec;
+
+// Velocity (3 components) and pressure, compact layout, pooled device memory.
+struct FieldPair {
+    PBuf u, q;
+    FieldPair() = default;
+    FieldPair(DevPool& pool, const Geom& g)
+        : u(pool, 2L * 3 * g.Ny * g.M), q(pool, 2L * g.Ny * g.M) {}
+};
+
+// Current state: the integration's own buffers, or the front of a multistep
+// history (which then holds the state, as after dnsalgo.cpp:207).
+struct StateRef {
+    double* u;
+    double* q;
+};
+
+// One NSE object (nse.cpp): flow constants, nonlinear-form state, device
+// scalars of the bulk-velocity solve, tau data per lambda_t.
+class NSECtx {
+   public:
+    NSECtx(Exec& ex, std::shared_ptr<FlowModel> model, const cfg_flags& f);
+    void reset_lambda(const std::vector<double>& lambda_t);
+    void nonlinear(const double* u, double* f, TickOps& ops);
+    void linear(const double* u, const double* q, double* out, TickOps& ops);
+    void solve(double* u, double* q, std::initializer_list<std::pair<double, const double*>> terms, int s,
+               TickOps& ops);
+    double dPdx();
+    double dPdz();
+    FlowModel& model() { return *model_; }
+    const cfg_flags& flags() const { return flags_; }
+
+   private:
+    void fetch_dP();
+    Exec& ex_;
+    std::shared_ptr<FlowModel> model_;
+    cfg_flags flags_;
+    int method_;
+    PBuf dP_;
+    bool dP_dirty_ = false;
+    double dPdx_ = 0, dPdz_ = 0;
+    std::vector<std::shared_ptr<TauStore>> tau_;
+};
+
+// A time-stepping algorithm of dnsalgo.cpp in tick form.
+class Stepper {
+   public:
+    Stepper(NSECtx& nse, const cfg_flags& f) : nse_(nse), flags_(f), t_(f.t0) {}
+    virtual ~Stepper() = default;
+    virtual bool full() const = 0;
+    virtual int ticks_per_step() const = 0;
+    virtual int Ninitsteps() const = 0;
+    virtual void tick(int sub, StateRef& cur, TickOps& ops) = 0;  // substep sub of one step
+    virtual void push(StateRef& cur, TickOps& ops) {}              // init-phase history push
+    virtual void reset_dt(double dt) = 0;
+    virtual FieldPair* front() { return nullptr; }  // multistep history front, order > 1
+    double time() const { return t_; }
+    void reset_time(double t) { t_ = t; }
+
+   protected:
+    NSECtx& nse_;
+    cfg_flags flags_;
+    double t_;
+};
+
+std::unique_ptr<Stepper> make_stepper(Exec& ex, NSECtx& nse, const cfg_flags& f);
+
+// One DNS integration (dns.cpp:22-163).
+class Integrator {
+   public:
+    Integrator(Exec& ex, const cfg_flags& flags, const double* u_spec, const double* q_spec, const double* Ubase,
+               const double* Wbase);
+    void next(TickOps& ops);           // produce the next tick (host state advances)
+    long steps() const { return steps_; }
+    double time() const { return main_->time(); }
+    StateRef state() const { return cur_; }
+    void set_state(const double* u_spec, const double* q_spec);
+    void get_state(double* u_spec, double* q_spec);
+    void reset_dt(double dt);
+    void reset_time(double t);
+    void reset_gradp(double dPdx, double dPdz);
+    void reset_bulkv(double Ubulk, double Wbulk);
+    double dPdx() { return main_nse_->dPdx(); }
+    double Ubulk() const { return model_->UbulkAct(); }
+    const FlowModel& model() const { return *model_; }
+    const cfg_flags& flags() const { return flags_; }
+    Exec& exec() { return ex_; }
+
+   private:
+    void own_state(TickOps* ops);  // make cur_ the integration's own buffers
+    Exec& ex_;
+    cfg_flags flags_;
+    std::shared_ptr<FlowModel> model_;
+    std::unique_ptr<NSECtx> main_nse_, init_nse_;
+    std::unique_ptr<Stepper> main_, init_;
+    FieldPair own_;
+    StateRef cur_{};
+    int init_sub_ = -1;  // -1: the next init-phase tick is the push
+    int main_sub_ = 0;
+    long steps_ = 0;
+};
+
+// Runs ticks of integrations on one grid, batched, on one stream.
+class Exec {
+   public:
+    explicit Exec(const cfg_grid& grid, bool dealias);
+    ~Exec();
+    Exec(const Exec&) = delete;
+    Exec& operator=(const Exec&) = delete;
+    void run(const std::vector<TickOps*>& ops);
+    void sync() { HIPCHK(hipStreamSynchronize(st)); }
+
+    std::shared_ptr<Geometry> geo;
+    hipStream_t st = nullptr;
+    DevPool pool;  // before everything that returns memory to it
+    TableCache cache;
+    KTimer kt;
+    long nfield = 0;  // doubles in one 3-component compact field
+    long nscal = 0;   // doubles in one 1-component compact field
+
+   private:
+    void reserve(int n);
+    int cap_ = 0;
+    DBuf yp_, xp_, zo_, fo_, lscr_;
+    std::vector<NLMember> nlt_;
+    std::vector<LinearMember> lt_;
+    std::vector<TauMember> tt_;
+};
+
+}  // namespace cfg

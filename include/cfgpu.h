@@ -110,6 +110,55 @@ int cfg_nonlinear(const cfg_grid* grid, const cfg_flags* flags, const double* u_
 int cfg_solve(const cfg_grid* grid, const cfg_flags* flags, double lambda_t, const double* u0_spec,
               const double* rhs_spec, double* u_spec, double* q_spec, double* dPdx);
 
+/* ---- Batched integrations ---------------------------------------------------
+ * Many independent DNS integrations on one device, of any grids. channelflow's
+ * callers run them one after another: edge-tracking bisection members
+ * (programs/edgetracking.cpp:472-530), multi-shooting segments
+ * (channelflow/multiShootingDSI.cpp:70-86), DSI Jacobian probes
+ * (channelflow/cfdsi.cpp:678-775). Members on the same grid advance together
+ * in batched launches; each member is a full DNS as above (own flags, state,
+ * time, init steps). A member's result does not depend on the others. */
+typedef struct cfg_batch cfg_batch;
+
+int cfg_batch_create(cfg_batch** out);
+int cfg_batch_destroy(cfg_batch* b);
+/* New member from spectral u (and q, may be NULL) in the reference layout, as
+ * cfg_dns_create; *member receives its id. */
+int cfg_batch_add(cfg_batch* b, const cfg_grid* grid, const cfg_flags* flags, const double* u_spec,
+                  const double* q_spec, int* member);
+int cfg_batch_remove(cfg_batch* b, int member);
+/* DNS::advance(fields, nsteps) for every member. */
+int cfg_batch_advance(cfg_batch* b, int nsteps);
+int cfg_batch_size(const cfg_batch* b, int* nmembers, int* nclasses);
+int cfg_batch_get_state(cfg_batch* b, int member, double* u_spec, double* q_spec);
+int cfg_batch_set_state(cfg_batch* b, int member, const double* u_spec, const double* q_spec);
+int cfg_batch_time(cfg_batch* b, int member, double* t);
+int cfg_batch_dpdx(cfg_batch* b, int member, double* dPdx);
+/* Kernel timing summed over the batch's streams (as cfg_dns_profile). */
+int cfg_batch_profile(cfg_batch* b, int enable);
+int cfg_batch_kernel_times(cfg_batch* b, double* total_ms, long long* counts);
+
+/* ---- Serving ----------------------------------------------------------------
+ * Continuous batching of integration requests (no counterpart in the
+ * reference). A request is (grid, flags, initial u and q, horizon in steps).
+ * Queued requests are admitted in order while their device memory fits the
+ * budget; every admitted request advances one tick (one nonlinear evaluation)
+ * per cfg_server_step together with the others; a request that reaches its
+ * horizon is retired at once and its result is copied back without stalling
+ * the rest. */
+typedef struct cfg_server cfg_server;
+
+int cfg_server_create(double budget_bytes, cfg_server** out);
+int cfg_server_destroy(cfg_server* s);
+int cfg_server_submit(cfg_server* s, const cfg_grid* grid, const cfg_flags* flags, const double* u_spec,
+                      const double* q_spec, int nsteps, long long* request);
+/* One server step. done receives up to max_done ids of requests whose results
+ * reached host memory since the last call (*ndone of them). */
+int cfg_server_step(cfg_server* s, long long* done, int max_done, int* ndone, int* nrunning, int* nqueued);
+/* Final state (reference layout) and time of a completed request; releases it. */
+int cfg_server_result(cfg_server* s, long long request, double* u_spec, double* q_spec, double* t);
+int cfg_server_memory(const cfg_server* s, double* used_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,127 @@
+"""Batched and served integrations on the GPU.
+
+Every member of a batch, and every request of the server, must give the same
+result as its own single-field DNS (the per-member kernels never mix
+members), and that result must stay within the 20-step bar of the oracle
+(relative L2 <= 1e-10 per component). A member's result must not depend on
+what else is in the batch or when it was admitted.
+"""
+import numpy as np
+import pytest
+
+import cfref as R
+import channelflow_amd as cfa
+from channelflow_amd.fields import random_field
+
+pytestmark = pytest.mark.gpu
+
+LX_C = 2 * np.pi / 1.14
+LZ_C = 2 * np.pi / 2.5
+
+COUETTE = dict(baseflow="LaminarBase", ulowerwall=-1.0, uupperwall=1.0, constraint="BulkVelocity", Ubulk=0.0,
+               nu=1 / 400, dt=0.02, timestepping="SBDF3", initstepping="SMRK2", nonlinearity="SkewSymmetric",
+               dealiasing="DealiasXZ")
+POISEUILLE = dict(baseflow="LaminarBase", constraint="PressureGradient", dPdx=-3.0 / 2000, nu=1 / 2000, dt=0.01,
+                  timestepping="SBDF3", initstepping="SMRK2", nonlinearity="Rotational", dealiasing="DealiasXZ")
+
+
+def case(i):
+    """Member i of the mixed test batch: grid, flags, initial field."""
+    grids = [(16, 17, 16), (24, 25, 24), (16, 17, 16), (24, 25, 24), (16, 33, 16), (16, 17, 16)]
+    Nx, Ny, Nz = grids[i % len(grids)]
+    fl = dict(COUETTE if i % 2 == 0 else POISEUILLE)
+    if i == 5:
+        fl.update(nu=1 / 300)  # same grid class, different tau data
+    if i == 4:
+        fl.update(timestepping="SBDF2")
+    u0 = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.05 + 0.02 * i, decay=0.4, seed=100 + i)
+    return (Nx, Ny, Nz), fl, u0
+
+
+def fields(grid, u0):
+    Nx, Ny, Nz = grid
+    return [cfa.FlowField(Nx, Ny, Nz, 3, LX_C, LZ_C, data=u0), cfa.FlowField(Nx, Ny, Nz, 1, LX_C, LZ_C)]
+
+
+def single(grid, fl, u0, n):
+    f = fields(grid, u0)
+    d = cfa.DNS(f, cfa.DNSFlags(**fl))
+    d.advance_resident(n)
+    u, q = d.state()
+    return u, q, d.time()
+
+
+def test_batch_matches_single_and_oracle(gpu):
+    n = 20
+    b = cfa.Batch()
+    ids = []
+    for i in range(6):
+        grid, fl, u0 = case(i)
+        ids.append(b.add(fields(grid, u0), cfa.DNSFlags(**fl)))
+    assert b.size() == (6, 3)
+    b.advance(n)
+    for i, mid in enumerate(ids):
+        grid, fl, u0 = case(i)
+        ub, qb = b.state(mid)
+        us, qs, ts = single(grid, fl, u0, n)
+        print(f"member {i} grid {grid} {fl['nonlinearity']} {fl['timestepping']}: batch vs single max|diff| "
+              f"{np.max(np.abs(ub - us)):.3e}", flush=True)
+        assert np.array_equal(ub, us) and np.array_equal(qb, qs)
+        assert b.time(mid) == pytest.approx(ts, abs=1e-14)
+        if i < 2:  # the oracle is slow; two members of different physics suffice
+            g = R.Grid(*grid, LX_C, LZ_C)
+            ref = R.OracleDNS(g, R.make_flags(**fl), u0, g.zeros(1))
+            ref.advance(n)
+            ur, _ = ref.state()
+            for c in range(3):
+                err = np.sqrt(R.l2dist2(g, ub[c:c + 1], ur[c:c + 1]) / R.l2norm2(g, ur[c:c + 1]))
+                print(f"  component {c}: rel L2 vs oracle {err:.3e}", flush=True)
+                assert err <= 1e-10
+
+
+def test_member_alone_and_in_mix(gpu):
+    grid, fl, u0 = case(1)
+    alone = cfa.Batch()
+    a = alone.add(fields(grid, u0), cfa.DNSFlags(**fl))
+    alone.advance(12)
+    ua, qa = alone.state(a)
+    mix = cfa.Batch()
+    for i in (0, 2, 3):
+        g2, f2, v2 = case(i)
+        mix.add(fields(g2, v2), cfa.DNSFlags(**f2))
+    m = mix.add(fields(grid, u0), cfa.DNSFlags(**fl))
+    mix.advance(5)
+    g3, f3, v3 = case(5)
+    mix.add(fields(g3, v3), cfa.DNSFlags(**f3))  # joins mid-run, in its init steps while the others are not
+    mix.advance(7)
+    um, qm = mix.state(m)
+    print("alone vs mix max|diff|", np.max(np.abs(ua - um)), flush=True)
+    assert np.array_equal(ua, um) and np.array_equal(qa, qm)
+
+
+def test_server_results_match_single(gpu):
+    horizons = [7, 3, 11, 5, 9, 4]
+    srv = cfa.Server(budget_bytes=8e6)  # about two requests at a time: admission must queue
+    rid = {}
+    for i, h in enumerate(horizons):
+        grid, fl, u0 = case(i)
+        rid[srv.submit(fields(grid, u0), cfa.DNSFlags(**fl), h)] = i
+    results = {}
+    max_running = 0
+    for _ in range(10000):
+        done, running, queued = srv.step()
+        max_running = max(max_running, running)
+        for r in done:
+            results[r] = srv.result(r)
+        if len(results) == len(horizons):
+            break
+    assert len(results) == len(horizons)
+    assert max_running >= 2
+    for r, (u, q, t) in results.items():
+        i = rid[r]
+        grid, fl, u0 = case(i)
+        us, qs, ts = single(grid, fl, u0, horizons[i])
+        print(f"request {i} horizon {horizons[i]}: served vs single max|diff| {np.max(np.abs(u - us)):.3e}",
+              flush=True)
+        assert np.array_equal(u, us) and np.array_equal(q, qs)
+        assert t == pytest.approx(ts, abs=1e-14)
