@@ -1,23 +1,30 @@
 """Benchmark of the MI355X DNS time step (one GPU).
 
-Workload (default): plane Poiseuille flow, Re_bulk ~ 2800 (Re_tau ~ 180 class),
-box Lx = 4 pi, Lz = 2 pi, grid 192 x 129 x 192, FP64, rotational nonlinearity,
-2/3 dealiasing, constant mean pressure gradient, SBDF3 with dt = 0.005 and the
-SMRK2 start, initial field = parabolic base flow + seeded divergence-free
-perturbation of L2 norm 0.2 (spectral decay 0.5). nu = 1/2800 and
-dPdx = -3 nu, the gradient whose laminar profile has bulk velocity 1
-(Re_bulk = 2800; centreline 1.5). The gradient -(180 nu)^2 of a turbulent
-Re_tau = 180 channel would drive a laminar centreline of 5.8 and put dt = 0.005
-past the explicit-term stability limit (CFL ~ 0.9).
+Default workload (the headline): a batched ensemble of 64 independent plane
+Couette integrations at Re = 400 on uneven grids -- 24 at 32x33x32, 24 at
+48x49x48, 16 at 64x65x64, box Lx = 2 pi/1.14, Lz = 2 pi/2.5, FP64,
+skew-symmetric nonlinearity, bulk velocity, 2/3 dealiasing, SBDF3 with
+dt = 0.02 and the SMRK2 start. Member m starts from the seeded
+divergence-free perturbation (coefficients scaled by 0.3^(|kx|+|kz|+ny/4))
+with an L2 norm drawn log-uniform in [1e-3, 0.3] from seed 2024 + m. A step
+is one DNS::advance step of every member; states, histories and tau data are
+resident in HBM before the timed region starts.
 
-A step is one DNS::advance step of that single field; the velocity, pressure
-and multistep history are resident in HBM before the timed region starts.
-Prints one JSON line (metric, value in grid-point*steps/s, roofline of the
-dominant kernel from HIP-event timings of the same timed region, and the
-oracle's CPU rate on a bounded sample of the same workload).
+The same JSON line also carries
+  plain_gpu   the plain single-field GPU path on the same members (one
+              integration at a time) and the batched/plain ratio;
+  served      a serving run: Poisson arrivals of Couette (Re 300-500) and
+              Poiseuille (Re_bulk 1500-3000) requests on grids 32^3 ... 128^3,
+              horizons 100-2000 steps, admitted by HBM budget, with p50/p99
+              completion latency and aggregate grid-point*steps/s;
+  single_c2   the plain 192x129x192 Poiseuille channel (rotational, SBDF3,
+              dt = 0.005) and its per-kernel times;
+  roofline    the dominant kernel of the headline workload (HIP events);
+  cpu_baseline the oracle (oracle/cpu_ref.cpp, one core) on a bounded sample.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -28,30 +35,73 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+LX_C, LZ_C = 2 * np.pi / 1.14, 2 * np.pi / 2.5
+METRIC = "grid-pt·steps/s batched+served vs plain GPU at 1e-10 rel-L2; p50/p99 latency"
+
+COUETTE = dict(baseflow="LaminarBase", ulowerwall=-1.0, uupperwall=1.0, constraint="BulkVelocity", Ubulk=0.0,
+               nu=1.0 / 400, timestepping="SBDF3", initstepping="SMRK2", nonlinearity="SkewSymmetric",
+               dealiasing="DealiasXZ", taucorrection=True, dt=0.02)
 
 
-def workload(name):
-    if name == "c2":
-        nu = 1.0 / 2800.0
-        return dict(
-            name="c2", label="plane Poiseuille Re_bulk=2800, 192x129x192, rotational NL, SBDF3 dt=0.005",
-            grid=(192, 129, 192), Lx=4 * np.pi, Lz=2 * np.pi, norm=0.2, decay=0.5, seed=2024,
-            flags=dict(baseflow="LaminarBase", constraint="PressureGradient", dPdx=-3.0 * nu, nu=nu,
-                       timestepping="SBDF3", initstepping="SMRK2", nonlinearity="Rotational",
-                       dealiasing="DealiasXZ", taucorrection=True, dt=0.005))
-    if name == "c1":
-        return dict(
-            name="c1", label="plane Couette Re=400, 32x33x32, skew-symmetric NL, bulk velocity, SBDF3 dt=0.02",
-            grid=(32, 33, 32), Lx=2 * np.pi / 1.14, Lz=2 * np.pi / 2.5, norm=0.1, decay=0.3, seed=2024,
-            flags=dict(baseflow="LaminarBase", ulowerwall=-1.0, uupperwall=1.0, constraint="BulkVelocity",
-                       Ubulk=0.0, nu=1.0 / 400, timestepping="SBDF3", initstepping="SMRK2",
-                       nonlinearity="SkewSymmetric", dealiasing="DealiasXZ", taucorrection=True, dt=0.02))
-    raise SystemExit(f"unknown workload {name}")
+def poiseuille(nu, dt):
+    """Laminar Poiseuille with bulk velocity 1 (centreline 1.5): dPdx = -3 nu."""
+    return dict(baseflow="LaminarBase", constraint="PressureGradient", dPdx=-3.0 * nu, nu=nu, timestepping="SBDF3",
+                initstepping="SMRK2", nonlinearity="Rotational", dealiasing="DealiasXZ", taucorrection=True, dt=dt)
 
 
-def kernel_bytes(wl, method):
-    """Compulsory HBM bytes per launch of each kernel (inputs read once, outputs written once)."""
-    Nx, Ny, Nz = wl["grid"]
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def c3_members(seed=2024):
+    """64 Couette members on three grids, norms log-uniform in [1e-3, 0.3]."""
+    grids = [(32, 33, 32)] * 24 + [(48, 49, 48)] * 24 + [(64, 65, 64)] * 16
+    out = []
+    for m, g in enumerate(grids):
+        r = np.random.default_rng(seed + m)
+        norm = math.exp(r.uniform(math.log(1e-3), math.log(0.3)))
+        out.append(dict(grid=g, flags=COUETTE, norm=norm, decay=0.3, seed=seed, member=m))
+    return out
+
+
+def c4_members(seed=2024):
+    """16 base fields at 32x49x32 (norm U[0.1, 0.2]), each with 8 copies
+    perturbed by 1e-7 times a unit-norm divergence-free field: 144 members."""
+    out = []
+    for b in range(16):
+        r = np.random.default_rng(seed + b)
+        norm = r.uniform(0.1, 0.2)
+        out.append(dict(grid=(32, 49, 32), flags=COUETTE, norm=norm, decay=0.3, seed=seed, member=b))
+        for k in range(8):
+            out.append(dict(grid=(32, 49, 32), flags=COUETTE, norm=norm, decay=0.3, seed=seed, member=b,
+                            delta=(seed + 7919, 1000 * b + k)))
+    return out
+
+
+def member_field(mem):
+    from channelflow_amd.fields import random_field
+    Nx, Ny, Nz = mem["grid"]
+    u = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=mem["norm"], decay=mem["decay"], seed=mem["seed"],
+                     member=mem["member"])
+    if "delta" in mem:
+        s, k = mem["delta"]
+        u = u + 1e-7 * random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=1.0, decay=mem["decay"], seed=s, member=k)
+    return u
+
+
+def flow_fields(cfa, grid, u):
+    Nx, Ny, Nz = grid
+    return [cfa.FlowField(Nx, Ny, Nz, 3, LX_C, LZ_C, data=u), cfa.FlowField(Nx, Ny, Nz, 1, LX_C, LZ_C)]
+
+
+def live(grid):
+    Nx, Ny, Nz = grid
+    return Ny * (2 * (Nx // 3) - 1) * (Nz // 3)  # complex values per component
+
+
+def kernel_bytes(grid, method):
+    """Compulsory HBM bytes of one member per launch of each kernel (inputs
+    read once, outputs written once), SBDF3 main steps."""
+    Nx, Ny, Nz = grid
     kxd, kzd = Nx // 3 - 1, Nz // 3 - 1
     M = (2 * kxd + 1) * (kzd + 1)
     L = 16.0 * Ny * M                     # one complex component array in the compact spectral layout
@@ -65,36 +115,274 @@ def kernel_bytes(wl, method):
         "k_zpass": nxout * X + nzout * X,
         "k_xfwd": nzout * X + nzout * L,
         "k_yfwd": nzout * L + 3 * L,
-        "k_combine": (2 * 3 * 3 + 3) * L,  # SBDF3: u_n..u_{n-2}, f_n..f_{n-2} in, rhs out
-        "k_tau_solve": (3 + 3 + 1) * L,     # rhs in, u and q out
-    }, L / 16.0
+        # SBDF3 right-hand side u_n..u_{n-2}, f_n..f_{n-2} read; u (3) and q (1) written
+        "k_tau_solve": (18 + 4) * L,
+    }
 
 
-def cpu_baseline(wl, steps_timed, budget_s=30.0):
-    """Oracle (oracle/cpu_ref.cpp, single thread) on the same workload: untimed
-    construction and SMRK2 start (dns.cpp:157-161), then a timed block of SBDF3
-    steps, as tools/benchmark.cpp:74-146 times channelflow."""
+# ---------------------------------------------------------------------------
+# runs
+# ---------------------------------------------------------------------------
+def run_batched(cfa, members, steps, warmup):
+    batch = cfa.Batch()
+    for mem in members:
+        batch.add(flow_fields(cfa, mem["grid"], member_field(mem)), cfa.DNSFlags(**mem["flags"]))
+    batch.advance(warmup)                 # SMRK2 start + warm-up SBDF3 steps, untimed
+    batch.profile(True)                   # HIP events around every launch on each class stream
+    t0 = time.perf_counter()
+    batch.advance(steps)                  # returns after every stream has drained
+    wall = time.perf_counter() - t0
+    kt = batch.kernel_times()
+    batch.profile(False)
+    for mid in range(0, len(members), max(1, len(members) // 8)):
+        u, _ = batch.state(mid)
+        if not np.all(np.isfinite(u)):
+            raise SystemExit(f"non-finite state of member {mid}")
+    pts = sum(np.prod(m["grid"]) for m in members)
+    return dict(value=pts * steps / wall, wall=wall, kt=kt, pts=pts, classes=batch.size()[1])
+
+
+def run_plain(cfa, members, steps, warmup):
+    """The plain single-field GPU path: one DNS per integration, one at a
+    time. Each grid class is timed on its first member; the ensemble's plain
+    time is the sum over members."""
+    per_class = {}
+    for mem in members:
+        key = (mem["grid"], tuple(sorted(mem["flags"].items())))
+        if key in per_class:
+            continue
+        dns = cfa.DNS(flow_fields(cfa, mem["grid"], member_field(mem)), cfa.DNSFlags(**mem["flags"]))
+        dns.advance_resident(warmup)
+        t0 = time.perf_counter()
+        dns.advance_resident(steps)
+        per_class[key] = (time.perf_counter() - t0) / steps
+    t_plain = sum(per_class[(m["grid"], tuple(sorted(m["flags"].items())))] for m in members)
+    pts = sum(np.prod(m["grid"]) for m in members)
+    return dict(value=pts / t_plain, ms_per_ensemble_step=t_plain * 1e3,
+                ms_per_step_by_grid={"x".join(map(str, k[0])): round(v * 1e3, 4) for k, v in per_class.items()})
+
+
+def serve_requests(n, seed):
+    """C5 request stream: request descriptions. The time step shrinks with the
+    grid (dt = dt_32 * 32 / Nx) so that the explicit advection number
+    kx_max U_max dt stays at its 32^3 value (0.23 Couette, 0.17 Poiseuille):
+    a fixed dt = 0.02 puts 128^3 Couette past the SBDF3 stability limit."""
+    r = np.random.default_rng(seed)
+    grids = [(32, 33, 32), (48, 49, 48), (64, 65, 64), (96, 97, 96), (128, 129, 128)]
+    reqs = []
+    for i in range(n):
+        g = grids[r.integers(len(grids))]
+        if r.uniform() < 0.5:
+            fl = dict(COUETTE, nu=1.0 / r.uniform(300, 500), dt=0.02 * 32 / g[0])
+        else:
+            fl = poiseuille(1.0 / r.uniform(1500, 3000), 0.01 * 32 / g[0])
+        reqs.append(dict(grid=g, flags=fl, norm=math.exp(r.uniform(math.log(1e-3), math.log(0.3))), decay=0.3,
+                         seed=seed, member=10000 + i, horizon=int(r.integers(100, 2001))))
+    return reqs
+
+
+def run_served(cfa, n, rate, seed, budget):
+    reqs = serve_requests(n, seed)
+    arrivals = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, n))
+    fields = [flow_fields(cfa, q["grid"], member_field(q)) for q in reqs]
+    srv = cfa.Server(budget)
+    rid_of = {}
+    submitted = 0
+    lat = {}
+    t0 = time.perf_counter()
+    max_running = 0
+    while len(lat) < n:
+        now = time.perf_counter() - t0
+        while submitted < n and arrivals[submitted] <= now:
+            q = reqs[submitted]
+            rid_of[srv.submit(fields[submitted], cfa.DNSFlags(**q["flags"]), q["horizon"])] = submitted
+            submitted += 1
+        done, running, queued = srv.step()
+        max_running = max(max_running, running)
+        now = time.perf_counter() - t0
+        for rid in done:
+            i = rid_of[rid]
+            u, _, _ = srv.result(rid)
+            if not np.all(np.isfinite(u)):
+                raise SystemExit(f"non-finite served result {i}")
+            lat[i] = now - arrivals[i]
+        if running == 0 and queued == 0 and submitted < n:
+            time.sleep(max(0.0, min(0.01, arrivals[submitted] - now)))
+    wall = time.perf_counter() - t0
+    work = sum(np.prod(q["grid"]) * q["horizon"] for q in reqs)
+    lats = np.array([lat[i] for i in range(n)])
+    return dict(value=work / wall, unit="grid-pt*steps/s", requests=n, arrival_rate_per_s=rate,
+                p50_latency_s=float(np.percentile(lats, 50)), p99_latency_s=float(np.percentile(lats, 99)),
+                wall_s=wall, max_concurrent=max_running, hbm_budget_bytes=budget)
+
+
+def run_single_c2(cfa, steps, warmup):
+    from channelflow_amd.fields import random_field
+    nu = 1.0 / 2800.0
+    Nx, Ny, Nz = 192, 129, 192
+    Lx, Lz = 4 * np.pi, 2 * np.pi
+    u0 = random_field(Nx, Ny, Nz, Lx, Lz, norm=0.2, decay=0.5, seed=2024)
+    fl = poiseuille(nu, 0.005)
+    dns = cfa.DNS([cfa.FlowField(Nx, Ny, Nz, 3, Lx, Lz, data=u0), cfa.FlowField(Nx, Ny, Nz, 1, Lx, Lz)],
+                  cfa.DNSFlags(**fl))
+    dns.advance_resident(warmup)
+    dns.profile(True)
+    t0 = time.perf_counter()
+    dns.advance_resident(steps)
+    wall = time.perf_counter() - t0
+    kt = dns.kernel_times()
+    dns.profile(False)
+    kb = kernel_bytes((Nx, Ny, Nz), "Rotational")
+    per = {}
+    for k, (ms, cnt) in kt.items():
+        if cnt > 0 and k in kb:
+            per[k] = dict(ms_per_step=round(ms / steps, 4), GBs=round(kb[k] * cnt / (ms * 1e-3) / 1e9, 1))
+    return dict(value=Nx * Ny * Nz * steps / wall, ms_per_step=wall / steps * 1e3, kernels=per,
+                workload="plane Poiseuille Re_bulk=2800, 192x129x192, rotational NL, SBDF3 dt=0.005")
+
+
+def cpu_baseline(members, steps_timed=2, budget_s=25.0):
+    """Oracle (oracle/cpu_ref.cpp, one core) on one member of each grid class:
+    untimed construction and SMRK2 start (dns.cpp:157-161), then timed SBDF3
+    steps, as tools/benchmark.cpp:74-146 times channelflow. The ensemble rate is
+    total grid points / sum over members of the class's time per step."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import cfref as R
-    from channelflow_amd.fields import random_field
-    Nx, Ny, Nz = wl["grid"]
-    g = R.Grid(Nx, Ny, Nz, wl["Lx"], wl["Lz"])
-    u0 = random_field(Nx, Ny, Nz, wl["Lx"], wl["Lz"], norm=wl["norm"], decay=wl["decay"], seed=wl["seed"])
-    dns = R.OracleDNS(g, R.make_flags(**wl["flags"]), u0, g.zeros(1))
-    dns.advance(2)  # SMRK2 start, untimed
-    t0 = time.perf_counter()
-    n = 0
-    while n < steps_timed:
-        dns.advance(1)
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    return dict(value=Nx * Ny * Nz * n / dt, unit="grid-pt*steps/s", cores=1, kind="port",
-                sample=f"{wl['name']} grid {Nx}x{Ny}x{Nz}: construction and 2 SMRK2 start steps untimed, "
-                       f"then {n} SBDF3 steps timed ({dt:.1f} s) on 1 core")
+    per = {}
+    spent = 0.0
+    for mem in members:
+        key = mem["grid"]
+        if key in per:
+            continue
+        g = R.Grid(*mem["grid"], LX_C, LZ_C)
+        ref = R.OracleDNS(g, R.make_flags(**mem["flags"]), member_field(mem), g.zeros(1))
+        ref.advance(2)
+        t0 = time.perf_counter()
+        n = 0
+        while n < steps_timed or (time.perf_counter() - t0) < 1.0:
+            ref.advance(1)
+            n += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        dt = time.perf_counter() - t0
+        spent += dt
+        per[key] = dt / n
+    t_ens = sum(per[m["grid"]] for m in members)
+    pts = sum(np.prod(m["grid"]) for m in members)
+    return dict(value=pts / t_ens, unit="grid-pt*steps/s", cores=1, kind="port",
+                sample="one member per grid class (" + ", ".join("x".join(map(str, k)) for k in per) +
+                       f"): construction and 2 SMRK2 start steps untimed, then >= {steps_timed} SBDF3 steps timed "
+                       f"({spent:.1f} s total) on 1 core; ensemble time = sum of members' class times")
 
 
+def roofline(members, kt, steps):
+    """Dominant kernel of the batched run: algorithmic bytes over its launches
+    in the timed region / their summed HIP-event durations."""
+    totals = {}
+    for mem in members:
+        for k, b in kernel_bytes(mem["grid"], mem["flags"]["nonlinearity"]).items():
+            totals[k] = totals.get(k, 0.0) + b * steps
+    dom = max((k for k in kt if kt[k][1] > 0 and k in totals), key=lambda k: kt[k][0])
+    ms, cnt = kt[dom]
+    achieved = totals[dom] / (ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_c3.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get(dom)
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "launches": cnt,
+            "avg_launch_ms": ms / cnt, "bytes_per_launch": totals[dom] / cnt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="c3", choices=["c3", "c4"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--serve-budget", type=float, default=64e9, help="HBM budget of the server, bytes")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run plain device copies of known size (for scaling HBM counters)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic_<workload>.json from two rocprofv3 --pmc runs and exit")
+    args = ap.parse_args()
+    if args.traffic_from:
+        cmd = ("rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -f csv -- python bench.py --workload "
+               f"{args.workload} --steps 5 --warmup 3 --no-cpu --no-serve --no-c2 --no-plain --copy-calibration")
+        traffic_from_counters(*args.traffic_from, os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json"),
+                              cmd)
+        return
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(ensembles shard by member across GPUs; see the sharding notes of the design)", file=sys.stderr)
+        sys.exit(2)
+
+    import channelflow_amd as cfa
+    members = c3_members() if args.workload == "c3" else c4_members()
+    label = ("C3 ensemble: 64 plane Couette Re=400 integrations (24x 32x33x32, 24x 48x49x48, 16x 64x65x64), "
+             "skew-symmetric NL, bulk velocity, SBDF3 dt=0.02" if args.workload == "c3" else
+             "C4 multi-shooting: 16 base fields x (1 + 8 perturbed copies), 144 plane Couette Re=400 "
+             "integrations at 32x49x32, skew-symmetric NL, bulk velocity, SBDF3 dt=0.02")
+    print(f"bench: {args.workload} batched, {len(members)} members, {args.steps} timed steps", file=sys.stderr,
+          flush=True)
+    b = run_batched(cfa, members, args.steps, args.warmup)
+    print(f"bench: batched {b['value']:.4g} grid-pt*steps/s, {b['wall'] / args.steps * 1e3:.3f} ms/step",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": METRIC,
+        "value": b["value"],
+        "unit": "grid-pt*steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": b["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded divergence-free perturbations)",
+        "config": {"workload": label, "members": len(members), "grid_classes": b["classes"],
+                   "grid_points_per_step": int(b["pts"]), "parallelism": "batched members, one GPU"},
+        "roofline": roofline(members, b["kt"], args.steps),
+        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in b["kt"].items() if v[1] > 0},
+    }
+    B_step = sum(16.0 * 25 * live(m["grid"]) for m in members)
+    out["step_roofline"] = {"bytes_per_step": B_step, "achieved": B_step / (b["wall"] / args.steps) / 1e9,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": B_step / (b["wall"] / args.steps) / 1e9 / HBM_PEAK_GBS}
+    if not args.no_plain:
+        print("bench: plain single-field path", file=sys.stderr, flush=True)
+        p = run_plain(cfa, members, max(20, args.steps // 4), args.warmup)
+        p["batched_over_plain"] = b["value"] / p["value"]
+        print(f"bench: plain {p['value']:.4g} grid-pt*steps/s ({p['ms_per_step_by_grid']})", file=sys.stderr,
+              flush=True)
+        out["plain_gpu"] = p
+    if not args.no_serve:
+        print("bench: serving", file=sys.stderr, flush=True)
+        out["served"] = run_served(cfa, args.serve_requests, args.serve_rate, 4242, args.serve_budget)
+        if "plain_gpu" in out:
+            out["served"]["served_over_plain"] = out["served"]["value"] / out["plain_gpu"]["value"]
+        print(f"bench: served {json.dumps(out['served'])}", file=sys.stderr, flush=True)
+    if not args.no_c2:
+        print("bench: single 192x129x192", file=sys.stderr, flush=True)
+        out["single_c2"] = run_single_c2(cfa, 50, 10)
+    if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(members)
+    if args.copy_calibration:
+        copy_calibration()
+    print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# HBM traffic from rocprofv3 counters
+# ---------------------------------------------------------------------------
 COPY_CAL_BYTES = 1 << 28
 
 
@@ -122,7 +410,7 @@ def copy_calibration(nbytes=COPY_CAL_BYTES, reps=3):
     chk(hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
     chk(hip.hipFree(a), "hipFree")
     chk(hip.hipFree(b), "hipFree")
-    print(f"copy calibration: {reps} device copies of {nbytes} bytes", flush=True)
+    print(f"copy calibration: {reps} device copies of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
 def _counter_rows(path, counter):
@@ -165,92 +453,6 @@ def traffic_from_counters(fetch_csv, write_csv, out_path, command):
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps({k: v for k, v in out.items() if not k.startswith("_")}))
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2")
-    ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run plain device copies of known size (for scaling HBM counters)")
-    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
-                    help="write profiles/traffic_<workload>.json from two rocprofv3 --pmc runs and exit")
-    args = ap.parse_args()
-    if args.traffic_from:
-        cmd = ("rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -f csv -- python bench.py --workload "
-               f"{args.workload} --steps 5 --warmup 3 --no-cpu --copy-calibration")
-        traffic_from_counters(*args.traffic_from, os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json"),
-                              cmd)
-        return
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(ensembles shard by member across GPUs; see the sharding notes of the design)", file=sys.stderr)
-        sys.exit(2)
-
-    import channelflow_amd as cfa
-    from channelflow_amd.fields import random_field
-    wl = workload(args.workload)
-    Nx, Ny, Nz = wl["grid"]
-    u0 = random_field(Nx, Ny, Nz, wl["Lx"], wl["Lz"], norm=wl["norm"], decay=wl["decay"], seed=wl["seed"])
-    flags = cfa.DNSFlags(**wl["flags"])
-    fu = cfa.FlowField(Nx, Ny, Nz, 3, wl["Lx"], wl["Lz"], data=u0)
-    fq = cfa.FlowField(Nx, Ny, Nz, 1, wl["Lx"], wl["Lz"])
-    dns = cfa.DNS([fu, fq], flags)
-    dns.advance_resident(args.warmup)       # SMRK2 start + warm-up SBDF3 steps, untimed
-
-    dns.profile(True)                       # HIP events around every launch on the DNS's stream
-    t0 = time.perf_counter()
-    dns.advance_resident(args.steps)        # returns after the stream has drained
-    t1 = time.perf_counter()
-    kt = dns.kernel_times()
-    dns.profile(False)
-    wall = t1 - t0
-    u, _ = dns.state()
-    if not np.all(np.isfinite(u)):
-        raise SystemExit("non-finite state after the timed steps")
-
-    pts = Nx * Ny * Nz
-    value = pts * args.steps / wall
-    kb, Lc = kernel_bytes(wl, wl["flags"]["nonlinearity"])
-    dom = max((k for k in kt if kt[k][1] > 0), key=lambda k: kt[k][0])
-    avg_s = kt[dom][0] / kt[dom][1] * 1e-3
-    achieved = kb[dom] / avg_s / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{wl['name']}.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(dom)
-    B_step = 16.0 * 25 * Lc                 # compulsory state bytes per SBDF3 step
-    out = {
-        "metric": "grid-pt·steps/s batched+served vs plain GPU at 1e-10 rel-L2; p50/p99 latency",
-        "value": value,
-        "unit": "grid-pt*steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (seeded divergence-free perturbation)",
-        "config": {"workload": wl["label"], "grid": list(wl["grid"]), "steps_per_call": args.steps,
-                   "parallelism": "single field, one GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
-        "step_roofline": {"bytes_per_step": B_step, "achieved": B_step / (wall / args.steps) / 1e9,
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": B_step / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
-        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1] > 0},
-    }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_steps)
-    if args.copy_calibration:
-        copy_calibration()
-    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
