@@ -15,6 +15,8 @@
 // reference's zeroPaddedModes() (flowfield.cpp:2235) is implicit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cfg_kernels.hpp"
 #include "cfg_math.hpp"
 #include "cfg_wave.hpp"
@@ -31,6 +33,13 @@ __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 imul(double s, double2 a) { return make_double2(-(s * a.y), s * a.x); }  // (i s) a
 
 __device__ __forceinline__ int kx_of(const Geom& g, int ikx) { return ikx <= g.kxd ? ikx : ikx - g.nkx; }
+
+// LDS layout of FFT lines: element i of a line sits at pidx(i) = i + i/8 (one
+// pad slot per 8 complex values spreads the power-of-two strides of the
+// Stockham passes over distinct banks), and consecutive lines are lpitch(N) =
+// N + N/8 + 1 slots apart (odd, so walks across lines are conflict-free too).
+__host__ __device__ constexpr int pidx(int i) { return i + (i >> 3); }
+__host__ __device__ constexpr int lpitch(int N) { return N + N / 8 + 1; }
 
 // ---------------------------------------------------------------------------
 // LDS Stockham FFTs over nlines contiguous lines of length N. sign -1 forward
@@ -68,7 +77,7 @@ template <int R, int SIGN>
 __device__ __forceinline__ void stockham_bfly(const double2* __restrict__ s, double2* __restrict__ d, const double2* tw,
                                               int j, int k, int NR, int NS, int tstep) {
     double2 v[4];
-    for (int r = 0; r < R; ++r) v[r] = s[j + r * NR];
+    for (int r = 0; r < R; ++r) v[r] = s[pidx(j + r * NR)];
     if (k != 0) {
         for (int r = 1; r < R; ++r) {
             double2 t = tw[r * k * tstep];
@@ -77,8 +86,8 @@ __device__ __forceinline__ void stockham_bfly(const double2* __restrict__ s, dou
         }
     }
     dft_small<R, SIGN>(v);
-    double2* o = d + (j / NS) * NS * R + k;
-    for (int r = 0; r < R; ++r) o[r * NS] = v[r];
+    const int o = (j / NS) * NS * R + k;
+    for (int r = 0; r < R; ++r) d[pidx(o + r * NS)] = v[r];
 }
 
 // runtime length (plan in P)
@@ -96,9 +105,10 @@ __device__ double2* fft_lds(double2* a, double2* b, const FFTPlan& P, const doub
             const int line = w / NR;
             const int j = w - line * NR;
             const int k = j % Ns;
-            if (R == 4) stockham_bfly<4, SIGN>(src + line * N, dst + line * N, tw, j, k, NR, Ns, tstep);
-            else if (R == 2) stockham_bfly<2, SIGN>(src + line * N, dst + line * N, tw, j, k, NR, Ns, tstep);
-            else stockham_bfly<3, SIGN>(src + line * N, dst + line * N, tw, j, k, NR, Ns, tstep);
+            const int lo = line * lpitch(N);
+            if (R == 4) stockham_bfly<4, SIGN>(src + lo, dst + lo, tw, j, k, NR, Ns, tstep);
+            else if (R == 2) stockham_bfly<2, SIGN>(src + lo, dst + lo, tw, j, k, NR, Ns, tstep);
+            else stockham_bfly<3, SIGN>(src + lo, dst + lo, tw, j, k, NR, Ns, tstep);
         }
         __syncthreads();
         double2* t = src;
@@ -149,7 +159,7 @@ __device__ __forceinline__ double2* fft_ct(double2* a, double2* b, const double2
         for (int w = threadIdx.x; w < nlines * NR; w += blockDim.x) {
             const int line = w / NR;
             const int j = w - line * NR;
-            stockham_bfly<R, SIGN>(a + line * N, b + line * N, tw, j, j % NS, NR, NS, TSTEP);
+            stockham_bfly<R, SIGN>(a + line * lpitch(N), b + line * lpitch(N), tw, j, j % NS, NR, NS, TSTEP);
         }
         __syncthreads();
         return fft_ct<N, SIGN, P + 1>(b, a, tw, nlines);
@@ -627,9 +637,10 @@ __global__ void __launch_bounds__(256) k_xinv(Geom g, FFTPlan px, int XL, int ns
     const bool deriv = fo >= nsrc;
     const int fs = deriv ? fo - nsrc : fo;
     const int nl = min(XL, nkz - ikz0);
+    const int P = lpitch(Nx);
     double2* tw = lds;
     double2* a = lds + Nx;
-    double2* b = a + XL * Nx;
+    double2* b = a + XL * P;
     stage_tw(tw, px);
     const double dk = 2 * PI / g.Lx;
     for (int w = threadIdx.x; w < nl * Nx; w += blockDim.x) {
@@ -642,14 +653,14 @@ __global__ void __launch_bounds__(256) k_xinv(Geom g, FFTPlan px, int XL, int ns
             v = yp[(((long)fs * Ny + y) * nkz + ikz0 + l) * nkx + ikx];
             if (deriv) v = imul(dk * kx_of(g, ikx), v);
         }
-        a[l * Nx + j] = v;
+        a[l * P + pidx(j)] = v;
     }
     __syncthreads();
     double2* r = fft_any<NX, +1>(a, b, px, tw, nl);
     double2* out = xp + (((long)fo * Ny + y) * Nx) * nkz + ikz0;
     for (int w = threadIdx.x; w < nl * Nx; w += blockDim.x) {
         const int x = w / nl, l = w - x * nl;
-        out[(long)x * nkz + l] = r[l * Nx + x];
+        out[(long)x * nkz + l] = r[l * P + pidx(x)];
     }
 }
 
@@ -665,14 +676,15 @@ __global__ void __launch_bounds__(256) k_xfwd(Geom g, FFTPlan px, int XL, const 
     const int y = blockIdx.x / nchunk, ikz0 = (blockIdx.x % nchunk) * XL;
     const int f = blockIdx.y;
     const int nl = min(XL, nkz - ikz0);
+    const int P = lpitch(Nx);
     double2* tw = lds;
     double2* a = lds + Nx;
-    double2* b = a + XL * Nx;
+    double2* b = a + XL * P;
     stage_tw(tw, px);
     const double2* in = zo + (((long)f * Ny + y) * Nx) * nkz + ikz0;
     for (int w = threadIdx.x; w < nl * Nx; w += blockDim.x) {
         const int x = w / nl, l = w - x * nl;
-        a[l * Nx + x] = in[(long)x * nkz + l];
+        a[l * P + pidx(x)] = in[(long)x * nkz + l];
     }
     __syncthreads();
     double2* r = fft_any<NX, -1>(a, b, px, tw, nl);
@@ -680,7 +692,7 @@ __global__ void __launch_bounds__(256) k_xfwd(Geom g, FFTPlan px, int XL, const 
     for (int w = threadIdx.x; w < nl * nkx; w += blockDim.x) {
         const int l = w / nkx, ikx = w - l * nkx;
         const int j = ikx <= g.kxd ? ikx : ikx - nkx + Nx;
-        const double2 v = r[l * Nx + j];
+        const double2 v = r[l * P + pidx(j)];
         fo[(((long)f * Ny + y) * nkz + ikz0 + l) * nkx + ikx] = make_double2(v.x * scale, v.y * scale);
     }
 }
@@ -706,9 +718,10 @@ __device__ void zc2r_pair(const ZCtx& c, const double2* __restrict__ xp, int fa,
     const Geom& g = c.g;
     const int Nz = g.Nz, nkz = g.nkz;
     const int nmid = Nz - 2 * nkz + 1;  // bins nkz .. Nz-nkz
+    const int P = lpitch(Nz);
     for (int w = threadIdx.x; w < c.nl * nmid; w += blockDim.x) {
         const int l = w / nmid, j = w - l * nmid;
-        c.wa[l * Nz + nkz + j] = make_double2(0.0, 0.0);
+        c.wa[l * P + pidx(nkz + j)] = make_double2(0.0, 0.0);
     }
     const long fsz = (long)g.Ny * g.Nx * nkz;
     const double dk = 2 * PI / g.Lz;
@@ -722,12 +735,12 @@ __device__ void zc2r_pair(const ZCtx& c, const double2* __restrict__ xp, int fa,
             B = xp[(long)fb * fsz + base];
             if (dzb) B = imul(dk * k, B);
         }
-        double2* z = c.wa + l * Nz;
+        double2* z = c.wa + l * P;
         if (k == 0) {
             z[0] = make_double2(A.x, B.x);
         } else {
-            z[k] = make_double2(A.x - B.y, A.y + B.x);       // A + iB
-            z[Nz - k] = make_double2(A.x + B.y, B.x - A.y);  // conj(A) + i conj(B)
+            z[pidx(k)] = make_double2(A.x - B.y, A.y + B.x);       // A + iB
+            z[pidx(Nz - k)] = make_double2(A.x + B.y, B.x - A.y);  // conj(A) + i conj(B)
         }
     }
     __syncthreads();
@@ -735,7 +748,8 @@ __device__ void zc2r_pair(const ZCtx& c, const double2* __restrict__ xp, int fa,
     double* pa = physf(c, da);
     double* pb = db >= 0 ? physf(c, db) : nullptr;
     for (int w = threadIdx.x; w < c.nl * Nz; w += blockDim.x) {
-        const double2 v = r[w];
+        const int l = w / Nz, zz = w - l * Nz;
+        const double2 v = r[l * P + pidx(zz)];
         pa[w] = v.x;
         if (pb) pb[w] = v.y;
     }
@@ -749,14 +763,18 @@ __device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int g
     const int Nz = g.Nz, nkz = g.nkz;
     const double* pa = physf(c, sa);
     const double* pb = sb >= 0 ? physf(c, sb) : nullptr;
-    for (int w = threadIdx.x; w < c.nl * Nz; w += blockDim.x) c.wa[w] = make_double2(pa[w], pb ? pb[w] : 0.0);
+    const int P = lpitch(Nz);
+    for (int w = threadIdx.x; w < c.nl * Nz; w += blockDim.x) {
+        const int l = w / Nz, zz = w - l * Nz;
+        c.wa[l * P + pidx(zz)] = make_double2(pa[w], pb ? pb[w] : 0.0);
+    }
     __syncthreads();
     double2* r = fft_any<NZ, -1>(c.wa, c.wb, c.pz, c.tw, c.nl);
     const long fsz = (long)g.Ny * g.Nx * nkz;
     for (int w = threadIdx.x; w < c.nl * nkz; w += blockDim.x) {
         const int l = w / nkz, k = w - l * nkz;
-        const double2 Zk = r[l * Nz + k];
-        const double2 Zn = r[l * Nz + ((Nz - k) % Nz)];
+        const double2 Zk = r[l * P + pidx(k)];
+        const double2 Zn = r[l * P + pidx((Nz - k) % Nz)];
         const long base = ((long)c.y * g.Nx + c.x0 + l) * nkz + k;
         zo[(long)ga * fsz + base] = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
         if (gb >= 0) zo[(long)gb * fsz + base] = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
@@ -786,8 +804,8 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, const
     c.XB = XB;
     c.tw = lds;
     c.wa = lds + g.Nz;
-    c.wb = c.wa + XB * g.Nz;
-    c.phys = reinterpret_cast<double*>(c.wb + XB * g.Nz);
+    c.wb = c.wa + XB * lpitch(g.Nz);
+    c.phys = reinterpret_cast<double*>(c.wb + XB * lpitch(g.Nz));
     stage_tw(c.tw, pz);
     const int Nz = g.Nz;
     const int npts = c.nl * Nz;
@@ -957,20 +975,31 @@ __global__ void __launch_bounds__(64) k_yfwd(Geom g, YMats ym, int have_g, const
 static size_t g_lds_budget = 64 * 1024;
 void set_lds_budget(size_t bytes) { g_lds_budget = bytes; }
 
-static int zpass_xb(const Geom& g, int nphys) {
-    const long per = (long)g.Nz * (32 + 8 * nphys);  // two complex work lines + nphys real lines
+// largest z-pass lines per block that fit the LDS budget (power of two <= 32)
+static int zpass_xb_max(const Geom& g, int nphys) {
+    const long per = 32L * lpitch(g.Nz) + 8L * g.Nz * nphys;  // two complex work lines + nphys real lines
     const long avail = (long)g_lds_budget - 16L * g.Nz;
     int xb = 1;
     while (xb < 32 && xb * 2 <= g.Nx && (long)(xb * 2) * per <= avail) xb *= 2;
     return xb;
 }
 
-static int xfft_lines(const Geom& g) {
-    const long per = 32L * g.Nx;  // two complex work lines
+// largest x-FFT lines per block that fit (power of two <= 16, at most the kz lines)
+static int xfft_lines_max(const Geom& g) {
+    const long per = 32L * lpitch(g.Nx);  // two complex work lines
     const long avail = (long)g_lds_budget - 16L * g.Nx;
-    int xl = 4;
-    while (xl < 16 && (long)(xl * 2) * per <= avail) xl *= 2;
+    int xl = 1;
+    while (xl < 16 && xl * 2 <= g.nkz && (long)(xl * 2) * per <= avail) xl *= 2;
     return xl;
+}
+
+static int zpass_xb(const Geom& g, int nphys, const NLTune& tn) {
+    const int m = zpass_xb_max(g, nphys);
+    return tn.zxb > 0 ? std::min(tn.zxb, m) : m;
+}
+static int xfft_lines(const Geom& g, const NLTune& tn) {
+    const int m = xfft_lines_max(g);
+    return tn.xl > 0 ? std::min(tn.xl, m) : std::min(4, m);
 }
 
 template <class K>
@@ -982,10 +1011,10 @@ static void allow_lds(K kernel, size_t bytes) {
 
 template <int NX>
 static void launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, int n, const double2* in,
-                     long sin, double2* out, long sout, hipStream_t st) {
-    const int XL = xfft_lines(g);
+                     long sin, double2* out, long sout, const NLTune& tn, hipStream_t st) {
+    const int XL = xfft_lines(g, tn);
     const int nchunk = (g.nkz + XL - 1) / XL;
-    const size_t lds = (size_t)(g.Nx + 2 * XL * g.Nx) * sizeof(double2);
+    const size_t lds = (size_t)(g.Nx + 2 * XL * lpitch(g.Nx)) * sizeof(double2);
     const dim3 grid(g.Ny * nchunk, nf, n);
     if (inverse) {
         allow_lds(k_xinv<NX>, lds);
@@ -998,22 +1027,23 @@ static void launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, i
 
 // in/out strides are per member, in double2
 static void launch_xfft(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, int n, const double2* in,
-                        long sin, double2* out, long sout, hipStream_t st) {
+                        long sin, double2* out, long sout, const NLTune& tn, hipStream_t st) {
     switch (g.Nx) {
 #define CASE_X(N) \
-    case N: launch_x<N>(inverse, g, px, nsrc, nf, n, in, sin, out, sout, st); return;
+    case N: launch_x<N>(inverse, g, px, nsrc, nf, n, in, sin, out, sout, tn, st); return;
         CFG_FFT_SIZES(CASE_X)
 #undef CASE_X
-        default: launch_x<0>(inverse, g, px, nsrc, nf, n, in, sin, out, sout, st);
+        default: launch_x<0>(inverse, g, px, nsrc, nf, n, in, sin, out, sout, tn, st);
     }
 }
 
 template <int METHOD, int NZ>
 static void launch_z_t(const Geom& g, const FFTPlan& pz, int n, const NLMember* tab, int want_g, const double2* xp,
-                       double2* zo, long sx, hipStream_t st) {
+                       double2* zo, long sx, const NLTune& tn, hipStream_t st) {
     const int nphys = (METHOD == NL_ROT) ? 6 : 9;
-    const int XB = zpass_xb(g, nphys);
-    const size_t lds = (size_t)g.Nz * sizeof(double2) + (size_t)XB * g.Nz * (2 * sizeof(double2) + nphys * sizeof(double));
+    const int XB = zpass_xb(g, nphys, tn);
+    const size_t lds = (size_t)g.Nz * sizeof(double2) +
+                       (size_t)XB * (2 * lpitch(g.Nz) * sizeof(double2) + (size_t)g.Nz * nphys * sizeof(double));
     const dim3 grid(g.Ny * ((g.Nx + XB - 1) / XB), n);
     allow_lds(k_zpass<METHOD, NZ>, lds);
     k_zpass<METHOD, NZ><<<grid, 256, lds, st>>>(g, pz, XB, tab, want_g, xp, zo, sx);
@@ -1021,20 +1051,31 @@ static void launch_z_t(const Geom& g, const FFTPlan& pz, int n, const NLMember* 
 
 template <int METHOD>
 static void launch_z(const Geom& g, const FFTPlan& pz, int n, const NLMember* tab, int want_g, const double2* xp,
-                     double2* zo, long sx, hipStream_t st) {
+                     double2* zo, long sx, const NLTune& tn, hipStream_t st) {
     switch (g.Nz) {
 #define CASE_Z(N) \
-    case N: launch_z_t<METHOD, N>(g, pz, n, tab, want_g, xp, zo, sx, st); return;
+    case N: launch_z_t<METHOD, N>(g, pz, n, tab, want_g, xp, zo, sx, tn, st); return;
         CFG_FFT_SIZES(CASE_Z)
 #undef CASE_Z
-        default: launch_z_t<METHOD, 0>(g, pz, n, tab, want_g, xp, zo, sx, st);
+        default: launch_z_t<METHOD, 0>(g, pz, n, tab, want_g, xp, zo, sx, tn, st);
+    }
+}
+
+static void launch_z_any(int method, const Geom& g, const FFTPlan& pz, int n, const NLMember* tab, int want_g,
+                         const double2* xp, double2* zo, long sx, const NLTune& tn, hipStream_t st) {
+    switch (method) {
+        case NL_ROT: launch_z<NL_ROT>(g, pz, n, tab, want_g, xp, zo, sx, tn, st); break;
+        case NL_CONV: launch_z<NL_CONV>(g, pz, n, tab, want_g, xp, zo, sx, tn, st); break;
+        case NL_SKEW: launch_z<NL_SKEW>(g, pz, n, tab, want_g, xp, zo, sx, tn, st); break;
+        default: launch_z<NL_DIV>(g, pz, n, tab, want_g, xp, zo, sx, tn, st);
     }
 }
 
 // Nonlinear term of n members of one grid class. rotating: some member has a
 // nonzero rotation (only then does the divergence form compute g).
 void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, int n,
-                      const NLMember* tab, bool rotating, const NLWork& w, hipStream_t st, KTimer* kt) {
+                      const NLMember* tab, bool rotating, const NLWork& w, const NLTune& tn, hipStream_t st,
+                      KTimer* kt) {
     KTimer dummy;
     KTimer& T = kt ? *kt : dummy;
     const dim3 ygrid((g.M + 63) / 64, (g.Ny + YT - 1) / YT, n);
@@ -1065,24 +1106,19 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     T.end(K_YINV, ev);
     ev = T.begin();
     // d/dx outputs (fields nsrc..) derive from fields 0..2 only
-    launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, n, yp, sy, xp, sx, st);
+    launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, n, yp, sy, xp, sx, tn, st);
     T.end(K_XINV, ev);
     const int want_g = (method != NL_DIV) || rotating;
     ev = T.begin();
-    switch (method) {
-        case NL_ROT: launch_z<NL_ROT>(g, pz, n, tab, want_g, xp, zo, sx, st); break;
-        case NL_CONV: launch_z<NL_CONV>(g, pz, n, tab, want_g, xp, zo, sx, st); break;
-        case NL_SKEW: launch_z<NL_SKEW>(g, pz, n, tab, want_g, xp, zo, sx, st); break;
-        default: launch_z<NL_DIV>(g, pz, n, tab, want_g, xp, zo, sx, st);
-    }
+    launch_z_any(method, g, pz, n, tab, want_g, xp, zo, sx, tn, st);
     T.end(K_ZPASS, ev);
     ev = T.begin();
     // forward x-FFT of the z-pass outputs that exist
-    if (want_g) launch_xfft(false, g, px, 0, 3, n, zo, sx, fo, sy, st);
+    if (want_g) launch_xfft(false, g, px, 0, 3, n, zo, sx, fo, sy, tn, st);
     if (method == NL_DIV || method == NL_SKEW) {
         const long off = 3L * g.Ny * g.Nx * g.nkz;
         const long offo = 3L * g.Ny * g.M;
-        launch_xfft(false, g, px, 0, 6, n, zo + off, sx, fo + offo, sy, st);
+        launch_xfft(false, g, px, 0, 6, n, zo + off, sx, fo + offo, sy, tn, st);
     }
     T.end(K_XFWD, ev);
     ev = T.begin();
@@ -1093,6 +1129,69 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
         default: k_yfwd<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, want_g, fo, sy, tab);
     }
     T.end(K_YFWD, ev);
+}
+
+// Autotuner of the nonlinear-term tilings: each candidate lines-per-block of
+// the z-pass and of the x-FFTs is timed (HIP events, best of 3 after a warm
+// launch) on the caller's work buffers and member table. The tiling only
+// changes which threads and blocks do a line's butterflies, never the
+// arithmetic, so every choice gives bit-identical results.
+NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, int n,
+                      const NLMember* tab, bool rotating, const NLWork& w, hipStream_t st) {
+    (void)ym;
+    const long sy = nlwork_y_doubles(g) / 2, sx = nlwork_x_doubles(g) / 2;
+    double2* yp = reinterpret_cast<double2*>(w.yp);
+    double2* xp = reinterpret_cast<double2*>(w.xp);
+    double2* zo = reinterpret_cast<double2*>(w.zo);
+    const int want_g = (method != NL_DIV) || rotating;
+    const int nphys = (method == NL_ROT) ? 6 : 9;
+    const int nxout = (method == NL_ROT) ? 6 : (method == NL_DIV ? 3 : 9);
+    const int nsrc = (method == NL_DIV) ? 3 : 6;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return NLTune{};
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return NLTune{};
+    }
+    auto timed = [&](auto&& launch) {
+        launch();
+        float best = 1e30f;
+        for (int r = 0; r < 3; ++r) {
+            (void)hipEventRecord(a, st);
+            launch();
+            (void)hipEventRecord(b, st);
+            (void)hipEventSynchronize(b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            best = std::min(best, ms);
+        }
+        return best;
+    };
+    NLTune out;
+    float bz = 1e30f;
+    for (int xb = 1; xb <= zpass_xb_max(g, nphys); xb *= 2) {
+        NLTune c;
+        c.zxb = xb;
+        const float ms = timed([&] { launch_z_any(method, g, pz, n, tab, want_g, xp, zo, sx, c, st); });
+        if (ms < bz) {
+            bz = ms;
+            out.zxb = xb;
+        }
+    }
+    float bx = 1e30f;
+    for (int xl = 1; xl <= xfft_lines_max(g); xl *= 2) {
+        NLTune c;
+        c.xl = xl;
+        const float ms = timed([&] { launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, n, yp, sy,
+                                                 xp, sx, c, st); });
+        if (ms < bx) {
+            bx = ms;
+            out.xl = xl;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return out;
 }
 
 }  // namespace cfg

@@ -195,7 +195,19 @@ struct NLWork {
 inline long nlwork_y_doubles(const Geom& g) { return 2L * 9 * g.Ny * g.M; }         // yp and fo
 inline long nlwork_x_doubles(const Geom& g) { return 2L * 9 * g.Ny * g.Nx * g.nkz; }  // xp and zo
 
+// Tile shapes of the nonlinear-term launches (0: the default rule).
+struct NLTune {
+    int zxb = 0;  // (y, x) lines per block of the z-pass
+    int xl = 0;   // kz lines per block of the x-FFTs
+};
+
 void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, int n,
-                      const NLMember* tab, bool rotating, const NLWork& w, hipStream_t st, KTimer* kt = nullptr);
+                      const NLMember* tab, bool rotating, const NLWork& w, const NLTune& tn, hipStream_t st,
+                      KTimer* kt = nullptr);
+
+// Times the candidate tilings on these work buffers (contents overwritten)
+// and returns the fastest. Results are identical for every tiling.
+NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, int n,
+                      const NLMember* tab, bool rotating, const NLWork& w, hipStream_t st);
 
 }  // namespace cfg

@@ -446,8 +446,15 @@ void Exec::run(const std::vector<TickOps*>& ops) {
                 rotating = rotating || o->nl.rotation != 0.0;
             }
         if (nlt_.empty()) continue;
-        launch_nonlinear(g, ym, geo->fx.plan, geo->fz.plan, m, (int)nlt_.size(), cache.get(nlt_, st), rotating, w,
-                         st, &kt);
+        const int n = (int)nlt_.size();
+        const NLMember* tab = cache.get(nlt_, st);
+        int nb = 1;
+        while (nb < n) nb *= 2;
+        auto tk = tune_.find({m, nb});
+        if (tk == tune_.end())  // first launch of this form at this batch size
+            tk = tune_.emplace(std::make_pair(m, nb),
+                               tune_nonlinear(g, ym, geo->fx.plan, geo->fz.plan, m, n, tab, rotating, w, st)).first;
+        launch_nonlinear(g, ym, geo->fx.plan, geo->fz.plan, m, n, tab, rotating, w, tk->second, st, &kt);
     }
     for (TickOps* o : ops)
         if (o->comb) {

@@ -167,6 +167,7 @@ class Exec {
     std::vector<NLMember> nlt_;
     std::vector<LinearMember> lt_;
     std::vector<TauMember> tt_;
+    std::map<std::pair<int, int>, NLTune> tune_;  // (nonlinear form, member-count bucket) -> tiling
 };
 
 }  // namespace cfg

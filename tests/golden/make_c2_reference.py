@@ -19,15 +19,16 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
 import cfref as R  # noqa: E402
-from bench import workload  # noqa: E402
 from channelflow_amd.fields import random_field  # noqa: E402
+from test_gpu_parity import C2_WORKLOAD  # noqa: E402
 
 MODES = [(0, 0), (1, 0), (-1, 0), (0, 1), (1, 1), (-1, 1), (2, 3), (-5, 7), (30, 10), (-63, 63), (63, 0), (17, 41)]
 STEPS = 20
 
 
 def main():
-    wl = workload("c2")
+    wl = dict(C2_WORKLOAD, label="plane Poiseuille Re_bulk=2800, 192x129x192, rotational NL, SBDF3 dt=0.005",
+              norm=0.2, decay=0.5, seed=2024)
     Nx, Ny, Nz = wl["grid"]
     g = R.Grid(Nx, Ny, Nz, wl["Lx"], wl["Lz"])
     u0 = random_field(Nx, Ny, Nz, wl["Lx"], wl["Lz"], norm=wl["norm"], decay=wl["decay"], seed=wl["seed"])

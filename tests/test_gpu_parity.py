@@ -41,8 +41,10 @@ def live_mask(Nx, Nz, kxd, kzd):
 
 @pytest.mark.parametrize("method", ["Rotational", "Convection", "Divergence", "SkewSymmetric"])
 @pytest.mark.parametrize("dealias", ["DealiasXZ", "NoDealiasing"])
-def test_nonlinear_single_call(gpu, method, dealias):
-    Nx, Ny, Nz = 24, 17, 18
+@pytest.mark.parametrize("grid", [(24, 17, 18), (32, 17, 48), (16, 19, 192)])
+def test_nonlinear_single_call(gpu, method, dealias, grid):
+    """Nz = 18 takes the runtime-length LDS z-pass, 48 and 192 the register one."""
+    Nx, Ny, Nz = grid
     g = R.Grid(Nx, Ny, Nz, LX_C, LZ_C)
     u = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.3, decay=0.5, seed=11, dealias=(dealias == "DealiasXZ"))
     fl = dict(nonlinearity=method, dealiasing=dealias, baseflow="LaminarBase", uupperwall=1.0, ulowerwall=-1.0,
@@ -54,7 +56,7 @@ def test_nonlinear_single_call(gpu, method, dealias):
     m = live_mask(Nx, Nz, kxd, kzd)
     fr = np.where(m[None, None], f_ref, 0)
     err = rel_l2_per_component(g, f_gpu, fr)
-    print(method, dealias, "rel L2 per component", err, flush=True)
+    print(grid, method, dealias, "rel L2 per component", err, flush=True)
     assert np.all(err <= 1e-12), err
 
 
@@ -144,18 +146,23 @@ def test_golden_trajectory(gpu):
     assert d <= t["tolerance_L2Dist"], d
 
 
+# 192x129x192 plane Poiseuille, Re_bulk = 2800 (nu = 1/2800, dPdx = -3 nu: laminar
+# bulk velocity 1), rotational, 2/3 dealiasing, SBDF3 dt = 0.005 with the SMRK2 start
+C2_WORKLOAD = dict(grid=(192, 129, 192), Lx=4 * np.pi, Lz=2 * np.pi,
+                   flags=dict(baseflow="LaminarBase", constraint="PressureGradient", dPdx=-3.0 / 2800, nu=1.0 / 2800,
+                              timestepping="SBDF3", initstepping="SMRK2", nonlinearity="Rotational",
+                              dealiasing="DealiasXZ", taucorrection=True, dt=0.005))
+
+
 def test_c2_full_size_20_steps(gpu):
     """192x129x192 Poiseuille (bench workload), 20 steps, against the oracle's committed
     fixture (tests/golden/make_c2_reference.py): per-component norms and every
     coefficient of 12 sampled modes at 1e-10; divergence-free and no-slip at full size."""
-    import sys
-    sys.path.insert(0, os.path.dirname(HERE))
-    from bench import workload
     from channelflow_amd.fields import divergence
     meta = json.load(open(os.path.join(HERE, "golden", "c2_20steps.json")))
     ref = np.fromfile(os.path.join(HERE, "golden", "c2_20steps_modes.c128"), dtype=np.complex128)
     ref = ref.reshape(meta["shape"])
-    wl = workload("c2")
+    wl = C2_WORKLOAD
     Nx, Ny, Nz = wl["grid"]
     u0 = random_field(Nx, Ny, Nz, wl["Lx"], wl["Lz"], norm=meta["norm"], decay=meta["decay"], seed=meta["seed"])
     fu = cfa.FlowField(Nx, Ny, Nz, 3, wl["Lx"], wl["Lz"], data=u0)
