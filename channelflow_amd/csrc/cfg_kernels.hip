@@ -3,13 +3,13 @@
 // Nonlinear term pipeline for one field (reference: NSE::nonlinear,
 // channelflow/nse.cpp:383-391 -> diffops.cpp rotationalNL/convectionNL/
 // divergenceNL/skewsymmetricNL):
-//   k_yinv   Chebyshev coefficients -> values at the Chebyshev points y_n,
+//   k_yinv_t Chebyshev coefficients -> values at the Chebyshev points y_n,
 //            with d/dy folded into the operator (T, T*D) and the base flow
 //            U(y), W(y), -Vsuck added to the kx=kz=0 mode;
 //   k_xinv   inverse x-FFT per (field, y, kz) line, optional d/dx;
 //   k_zpass  per (y, x) line: c2r z-FFTs, pointwise products, r2c z-FFTs;
 //   k_xfwd   forward x-FFT, live kx kept, 1/(Nx Nz) scaling;
-//   k_yfwd   values -> Chebyshev coefficients, with the spectral divergence
+//   k_yfwd_t values -> Chebyshev coefficients, with the spectral divergence
 //            of u_i u_j folded in for the divergence/skew-symmetric forms.
 // Only non-aliased modes |kx| <= kxd, kz <= kzd are ever stored, so the
 // reference's zeroPaddedModes() (flowfield.cpp:2235) is implicit.
@@ -235,45 +235,63 @@ __device__ __forceinline__ CV cview(double* base, const Geom& g, int comp, int m
 // TauSolver::solve, tausolver.cpp:347-450, with solve_P_and_v :193-251).
 constexpr int TAU_WPB = 8;  // waves (modes) per block; 8 complex doubles = one 128-byte row
 
+// solve_P_and_v for the real and imaginary parts at once (NR = 2 right-hand
+// sides per scan); per part the same arithmetic as the one-RHS routines
+// (tausolver.cpp:193-251).
 template <int K>
-__device__ __forceinline__ void tau_P_and_v_wave(const WCtx& c, const double* cm, const double* sc, double kappa2, double lambda,
-                                 double nu, bool taucorr, const WV<K>& r, const WV<K>& Ry, WV<K>& P, WV<K>& v) {
+__device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, const double* cm, const double* sc, double kappa2,
+                                                  double lambda, double nu, bool taucorr, const WV<K> (&r)[2],
+                                                  const WV<K> (&Ry)[2], WV<K> (&P)[2], WV<K> (&v)[2]) {
     const long Ny = c.N;
-    P = whelm<K>(r, cm + 0 * Ny, cm + 1 * Ny, cm + 2 * Ny, kappa2, 0.0, 0.0, c);
-    WV<K> tmp = wdiff<K>(P, c);
-    for (int k = 0; k < K; ++k) {
-        tmp.e[k] -= Ry.e[k];
-        tmp.o[k] -= Ry.o[k];
-    }
-    v = whelm<K>(tmp, cm + 3 * Ny, cm + 4 * Ny, cm + 5 * Ny, lambda, 0.0, 0.0, c);
+    const double z2[2] = {0.0, 0.0};
+    whelm_n<K, 2>(r, cm + 0 * Ny, cm + 1 * Ny, cm + 2 * Ny, kappa2, z2, z2, c, P);
+    WV<K> tmp[2];
+    wdiff_n<K, 2>(P, c, tmp);
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            tmp[p].e[k] -= Ry[p].e[k];
+            tmp[p].o[k] -= Ry[p].o[k];
+        }
+    whelm_n<K, 2>(tmp, cm + 3 * Ny, cm + 4 * Ny, cm + 5 * Ny, lambda, z2, z2, c, v);
     // influence-matrix correction (tausolver.cpp:178-191)
-    double dvb, dva;
-    wends<K>(wdiff<K>(v, c), c, dvb, dva);
-    const double dplus = -sc[0] * dvb - sc[1] * dva;
-    const double dminus = -sc[2] * dvb - sc[3] * dva;
+    double dvb[2], dva[2];
+    wdiff_n<K, 2>(v, c, tmp);
+    wends_n<K, 2>(tmp, c, dvb, dva);
     {
         const WV<K> Pp = wload<K>(cm + 6 * Ny, c), Pm = wload<K>(cm + 7 * Ny, c);
         const WV<K> vp = wload<K>(cm + 8 * Ny, c), vm = wload<K>(cm + 9 * Ny, c);
-        for (int k = 0; k < K; ++k) {
-            P.e[k] += dplus * Pp.e[k] + dminus * Pm.e[k];
-            P.o[k] += dplus * Pp.o[k] + dminus * Pm.o[k];
-            v.e[k] += dplus * vp.e[k] + dminus * vm.e[k];
-            v.o[k] += dplus * vp.o[k] + dminus * vm.o[k];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const double dplus = -sc[0] * dvb[p] - sc[1] * dva[p];
+            const double dminus = -sc[2] * dvb[p] - sc[3] * dva[p];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                P[p].e[k] += dplus * Pp.e[k] + dminus * Pm.e[k];
+                P[p].o[k] += dplus * Pp.o[k] + dminus * Pm.o[k];
+                v[p].e[k] += dplus * vp.e[k] + dminus * vm.e[k];
+                v[p].o[k] += dplus * vp.o[k] + dminus * vm.o[k];
+            }
         }
     }
     if (!taucorr) return;
     const int Nb = c.Nb;
-    double sigma1_Nb = lambda * wtop<K>(v, c) - nu * 0.0 - wtop<K>(Ry, c);
-    double sigma1_Nb1 = lambda * wtop1<K>(v, c) - nu * 0.0 - wtop1<K>(Ry, c);
-    sigma1_Nb1 += (4.0 / c.L) * Nb * wtop<K>(P, c);
-    const double sigmaNb = sigma1_Nb / (1.0 - sc[5]);
-    const double sigmaNb1 = sigma1_Nb1 / (1.0 - sc[4]);
     const WV<K> P0 = wload<K>(cm + 10 * Ny, c), v0 = wload<K>(cm + 11 * Ny, c);
-    for (int k = 0; k < K; ++k) {
-        P.e[k] += sigmaNb1 * P0.e[k];
-        P.o[k] += sigmaNb * P0.o[k];
-        v.e[k] += sigmaNb * v0.e[k];
-        v.o[k] += sigmaNb1 * v0.o[k];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+        double sigma1_Nb = lambda * wtop<K>(v[p], c) - nu * 0.0 - wtop<K>(Ry[p], c);
+        double sigma1_Nb1 = lambda * wtop1<K>(v[p], c) - nu * 0.0 - wtop1<K>(Ry[p], c);
+        sigma1_Nb1 += (4.0 / c.L) * Nb * wtop<K>(P[p], c);
+        const double sigmaNb = sigma1_Nb / (1.0 - sc[5]);
+        const double sigmaNb1 = sigma1_Nb1 / (1.0 - sc[4]);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            P[p].e[k] += sigmaNb1 * P0.e[k];
+            P[p].o[k] += sigmaNb * P0.o[k];
+            v[p].e[k] += sigmaNb * v0.e[k];
+            v[p].o[k] += sigmaNb1 * v0.o[k];
+        }
     }
 }
 
@@ -364,19 +382,21 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const T
             Pi = whelm<K>(wdiff<K>(Ryi, c), cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
             vr = wzero<K>();
             vi = wzero<K>();
-        } else {
-            WV<K> rr = wdiff<K>(Ryr, c);
+        } else {  // real and imaginary parts together
+            const WV<K> Ry2[2] = {Ryr, Ryi};
+            WV<K> rr[2], P2[2], v2[2];
+            wdiff_n<K, 2>(Ry2, c, rr);
             for (int k = 0; k < K; ++k) {
-                rr.e[k] -= kx2pi * Rxi.e[k] + kz2pi * Rzi.e[k];
-                rr.o[k] -= kx2pi * Rxi.o[k] + kz2pi * Rzi.o[k];
+                rr[0].e[k] -= kx2pi * Rxi.e[k] + kz2pi * Rzi.e[k];
+                rr[0].o[k] -= kx2pi * Rxi.o[k] + kz2pi * Rzi.o[k];
+                rr[1].e[k] += kx2pi * Rxr.e[k] + kz2pi * Rzr.e[k];
+                rr[1].o[k] += kx2pi * Rxr.o[k] + kz2pi * Rzr.o[k];
             }
-            tau_P_and_v_wave<K>(c, cm, sc, kappa2, lambda, nu, taucorr != 0, rr, Ryr, Pr, vr);
-            rr = wdiff<K>(Ryi, c);
-            for (int k = 0; k < K; ++k) {
-                rr.e[k] += kx2pi * Rxr.e[k] + kz2pi * Rzr.e[k];
-                rr.o[k] += kx2pi * Rxr.o[k] + kz2pi * Rzr.o[k];
-            }
-            tau_P_and_v_wave<K>(c, cm, sc, kappa2, lambda, nu, taucorr != 0, rr, Ryi, Pi, vi);
+            tau_P_and_v_wave2<K>(c, cm, sc, kappa2, lambda, nu, taucorr != 0, rr, Ry2, P2, v2);
+            Pr = P2[0];
+            Pi = P2[1];
+            vr = v2[0];
+            vi = v2[1];
         }
         if (zero) {  // real parts only (nse.cpp:555-563)
             Pi = wzero<K>();
@@ -421,8 +441,26 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const T
             xi = zero ? wzero<K>() : whelm<K>(fi, vb, vu, vd, lambda, 0.0, 0.0, c);
             wstore_c<K>(lds + (w * 3 + slot) * Ny, c, xr, xi);
         };
-        solve_uw(kx2pi, Rxr, Rxi, zc.umean, zc.dPdx_out, 0);
-        solve_uw(kz2pi, Rzr, Rzi, zc.wmean, zc.dPdz_out, 1);
+        if (zero) {
+            solve_uw(kx2pi, Rxr, Rxi, zc.umean, zc.dPdx_out, 0);
+            solve_uw(kz2pi, Rzr, Rzi, zc.wmean, zc.dPdz_out, 1);
+        } else {  // u and w, real and imaginary parts: four right-hand sides of one operator
+            WV<K> f4[4], x4[4];
+            for (int k = 0; k < K; ++k) {
+                f4[0].e[k] = -(kx2pi * Pi.e[k]) - Rxr.e[k];
+                f4[0].o[k] = -(kx2pi * Pi.o[k]) - Rxr.o[k];
+                f4[1].e[k] = kx2pi * Pr.e[k] - Rxi.e[k];
+                f4[1].o[k] = kx2pi * Pr.o[k] - Rxi.o[k];
+                f4[2].e[k] = -(kz2pi * Pi.e[k]) - Rzr.e[k];
+                f4[2].o[k] = -(kz2pi * Pi.o[k]) - Rzr.o[k];
+                f4[3].e[k] = kz2pi * Pr.e[k] - Rzi.e[k];
+                f4[3].o[k] = kz2pi * Pr.o[k] - Rzi.o[k];
+            }
+            const double z4[4] = {0.0, 0.0, 0.0, 0.0};
+            whelm_n<K, 4>(f4, vb, vu, vd, lambda, z4, z4, c, x4);
+            wstore_c<K>(lds + (w * 3 + 0) * Ny, c, x4[0], x4[1]);
+            wstore_c<K>(lds + (w * 3 + 1) * Ny, c, x4[2], x4[3]);
+        }
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < 2 * Ny * TAU_WPB; idx += nthr) {
@@ -550,78 +588,6 @@ void launch_linear(const Geom& g, int n, const LinearMember* tab, hipStream_t st
 // ===========================================================================
 // Nonlinear term
 // ===========================================================================
-constexpr int YT = 8;  // y outputs per thread in the y-transforms
-
-// Chebyshev -> y values. Outputs per method:
-//   ROT : u v w wx wy wz      CONV/SKEW : u v w du/dy dv/dy dw/dy      DIV : u v w
-template <int METHOD>
-__global__ void __launch_bounds__(64) k_yinv(Geom g, YMats ym, const NLMember* __restrict__ tab, double2* __restrict__ yp0,
-                                             long sy) {
-    const NLMember& me = tab[blockIdx.z];
-    const double2* __restrict__ u = reinterpret_cast<const double2*>(me.u);
-    double2* __restrict__ yp = yp0 + blockIdx.z * sy;
-    const double* __restrict__ base = me.base;
-    const double Vsuck = me.Vsuck;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y0 = blockIdx.y * YT;
-    if (m >= g.M) return;
-    const int Ny = g.Ny;
-    const long M = g.M;
-    constexpr bool Q0 = (METHOD != NL_DIV), Q1 = (METHOD == NL_CONV || METHOD == NL_SKEW), Q2 = (METHOD != NL_DIV);
-    double2 P[3][YT], Q[3][YT];
-    for (int c = 0; c < 3; ++c)
-        for (int j = 0; j < YT; ++j) P[c][j] = Q[c][j] = make_double2(0.0, 0.0);
-    for (int k = 0; k < Ny; ++k) {
-        const double2 a0 = u[(0L * Ny + k) * M + m];
-        const double2 a1 = u[(1L * Ny + k) * M + m];
-        const double2 a2 = u[(2L * Ny + k) * M + m];
-#pragma unroll
-        for (int j = 0; j < YT; ++j) {
-            const int y = min(y0 + j, Ny - 1);
-            const double t = ym.T[(long)y * Ny + k];
-            P[0][j].x += t * a0.x; P[0][j].y += t * a0.y;
-            P[1][j].x += t * a1.x; P[1][j].y += t * a1.y;
-            P[2][j].x += t * a2.x; P[2][j].y += t * a2.y;
-            if (Q0 || Q1 || Q2) {
-                const double td = ym.TD[(long)y * Ny + k];
-                if (Q0) { Q[0][j].x += td * a0.x; Q[0][j].y += td * a0.y; }
-                if (Q1) { Q[1][j].x += td * a1.x; Q[1][j].y += td * a1.y; }
-                if (Q2) { Q[2][j].x += td * a2.x; Q[2][j].y += td * a2.y; }
-            }
-        }
-    }
-    const int ikx = m % g.nkx, ikz = m / g.nkx;
-    const double dx = 2 * PI * kx_of(g, ikx) / g.Lx, dz = 2 * PI * ikz / g.Lz;
-    for (int j = 0; j < YT; ++j) {
-        const int y = y0 + j;
-        if (y >= Ny) break;
-        if (m == 0) {  // base flow on the kx = kz = 0 mode (nse.cpp:28-36)
-            P[0][j].x += base[0 * Ny + y];
-            Q[0][j].x += base[1 * Ny + y];
-            P[2][j].x += base[2 * Ny + y];
-            Q[2][j].x += base[3 * Ny + y];
-            P[1][j].x -= Vsuck;
-        }
-        double2 o[6];
-        o[0] = P[0][j];
-        o[1] = P[1][j];
-        o[2] = P[2][j];
-        int nout = 3;
-        if (METHOD == NL_ROT) {
-            o[3] = csub(Q[2][j], imul(dz, P[1][j]));                // dw/dy - dv/dz
-            o[4] = csub(imul(dz, P[0][j]), imul(dx, P[2][j]));      // du/dz - dw/dx
-            o[5] = csub(imul(dx, P[1][j]), Q[0][j]);                // dv/dx - du/dy
-            nout = 6;
-        } else if (METHOD == NL_CONV || METHOD == NL_SKEW) {
-            o[3] = Q[0][j];
-            o[4] = Q[1][j];
-            o[5] = Q[2][j];
-            nout = 6;
-        }
-        for (int f = 0; f < nout; ++f) yp[((long)f * Ny + y) * M + m] = o[f];
-    }
-}
-
 // inverse x-FFT: yp [F][Ny][nkz][nkx] -> xp [Fo][Ny][Nx][nkz]; output f >= nsrc is
 // d/dx of source field f - nsrc. XL kz-lines per block.
 template <int NX>
@@ -910,64 +876,215 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, const
     }
 }
 
-// y values -> Chebyshev coefficients, f_i = Ti G_i + DTi H_i with
-//   ROT/CONV : G_i = g_i,                                    H_i = 0
-//   SKEW     : G_i = g_i + 0.5 (ikx uu_i0 + ikz uu_i2),        H_i = 0.5 uu_i1
-//   DIV      : G_i = [g_i] + (ikx uu_i0 + ikz uu_i2),          H_i = uu_i1
+// ---- y transforms as LDS-tiled contractions ----------------------------------
+// Both directions are dense Ny x Ny products over the Chebyshev index applied to
+// every (component, mode) column. A block computes YB_M modes x YB_R output
+// rows of one member; the contraction index runs through LDS in chunks of
+// YB_K, and each thread accumulates a 2 x 2 (mode, row) micro-tile of every
+// product it needs, so the epilogue that mixes components (vorticity, base
+// flow, divergence terms) stays in registers. Row tiles of one mode tile are
+// adjacent in launch order, so the input tile's re-reads come from L2.
+constexpr int YB_M = 32, YB_R = 32, YB_K = 16;
+
+// Chebyshev -> y values (same outputs as k_yinv). QM: which components need T*D.
 template <int METHOD>
-__global__ void __launch_bounds__(64) k_yfwd(Geom g, YMats ym, int have_g, const double2* __restrict__ fo0, long sy,
-                                             const NLMember* __restrict__ tab) {
-    const double2* __restrict__ fo = fo0 + blockIdx.z * sy;
-    double2* __restrict__ f = reinterpret_cast<double2*>(tab[blockIdx.z].f);
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k0 = blockIdx.y * YT;
-    if (m >= g.M) return;
+__global__ void __launch_bounds__(256) k_yinv_t(Geom g, YMats ym, const NLMember* __restrict__ tab,
+                                                double2* __restrict__ yp0, long sy) {
+    constexpr bool Q0 = (METHOD != NL_DIV), Q1 = (METHOD == NL_CONV || METHOD == NL_SKEW), Q2 = (METHOD != NL_DIV);
+    constexpr bool QA = Q0 || Q1 || Q2;
+    __shared__ double2 Xs[3][YB_K][YB_M];
+    __shared__ double As[2][YB_R][YB_K + 1];
+    const NLMember& me = tab[blockIdx.y];
+    const double2* __restrict__ u = reinterpret_cast<const double2*>(me.u);
+    double2* __restrict__ yp = yp0 + blockIdx.y * sy;
     const int Ny = g.Ny;
     const long M = g.M;
-    const int ikx = m % g.nkx, ikz = m / g.nkx;
-    const double dx = 2 * PI * kx_of(g, ikx) / g.Lx, dz = 2 * PI * ikz / g.Lz;
-    constexpr bool DIVF = (METHOD == NL_DIV || METHOD == NL_SKEW);
-    const double beta = (METHOD == NL_SKEW) ? 0.5 : 1.0;
-    // symmetric uu index -> fo field: 00:3 01:4 02:5 11:6 12:7 22:8
-    const int uuf[3][3] = {{3, 4, 5}, {4, 6, 7}, {5, 7, 8}};
-    double2 A[3][YT];
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < YT; ++j) A[i][j] = make_double2(0.0, 0.0);
-    for (int y = 0; y < Ny; ++y) {
-        double2 G[3], H[3];
-        for (int i = 0; i < 3; ++i) {
-            G[i] = have_g ? fo[((long)i * Ny + y) * M + m] : make_double2(0.0, 0.0);
-            H[i] = make_double2(0.0, 0.0);
-            if (DIVF) {
-                const double2 x0 = fo[((long)uuf[i][0] * Ny + y) * M + m];
-                const double2 x1 = fo[((long)uuf[i][1] * Ny + y) * M + m];
-                const double2 x2 = fo[((long)uuf[i][2] * Ny + y) * M + m];
-                const double2 d = cadd(imul(dx, x0), imul(dz, x2));
-                G[i] = make_double2(G[i].x + beta * d.x, G[i].y + beta * d.y);
-                H[i] = make_double2(beta * x1.x, beta * x1.y);
-            }
-        }
+    const int nrt = (Ny + YB_R - 1) / YB_R;
+    const int m0 = (blockIdx.x / nrt) * YB_M, y0 = (blockIdx.x % nrt) * YB_R;
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    double2 P[3][2][2], Q[3][2][2];
 #pragma unroll
-        for (int j = 0; j < YT; ++j) {
-            const int k = min(k0 + j, Ny - 1);
-            const double t = ym.Ti[(long)k * Ny + y];
-            for (int i = 0; i < 3; ++i) {
-                A[i][j].x += t * G[i].x;
-                A[i][j].y += t * G[i].y;
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) P[c][i][j] = Q[c][i][j] = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < Ny; k0 += YB_K) {
+        for (int i = threadIdx.x; i < 3 * YB_K * YB_M; i += 256) {
+            const int mm = i % YB_M, kk = (i / YB_M) % YB_K, c = i / (YB_M * YB_K);
+            const int k = k0 + kk, m = m0 + mm;
+            Xs[c][kk][mm] = (k < Ny && m < M) ? u[((long)c * Ny + k) * M + m] : make_double2(0.0, 0.0);
+        }
+        for (int i = threadIdx.x; i < (QA ? 2 : 1) * YB_R * YB_K; i += 256) {
+            const int kk = i % YB_K, r = (i / YB_K) % YB_R, a = i / (YB_K * YB_R);
+            const int k = k0 + kk, y = y0 + r;
+            As[a][r][kk] = (k < Ny && y < Ny) ? (a == 0 ? ym.T : ym.TD)[(long)y * Ny + k] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < YB_K; ++kk) {
+            double t[2], d[2];
+            double2 x[3][2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                t[j] = As[0][ty + 16 * j][kk];
+                d[j] = QA ? As[1][ty + 16 * j][kk] : 0.0;
             }
-            if (DIVF) {
-                const double td = ym.DTi[(long)k * Ny + y];
-                for (int i = 0; i < 3; ++i) {
-                    A[i][j].x += td * H[i].x;
-                    A[i][j].y += td * H[i].y;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int i = 0; i < 2; ++i) x[c][i] = Xs[c][kk][tx + 16 * i];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        P[c][i][j].x += t[j] * x[c][i].x;
+                        P[c][i][j].y += t[j] * x[c][i].y;
+                    }
+                    if (Q0) { Q[0][i][j].x += d[j] * x[0][i].x; Q[0][i][j].y += d[j] * x[0][i].y; }
+                    if (Q1) { Q[1][i][j].x += d[j] * x[1][i].x; Q[1][i][j].y += d[j] * x[1][i].y; }
+                    if (Q2) { Q[2][i][j].x += d[j] * x[2][i].x; Q[2][i][j].y += d[j] * x[2][i].y; }
                 }
+        }
+        __syncthreads();
+    }
+    const double* __restrict__ base = me.base;
+    const double Vsuck = me.Vsuck;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int m = m0 + tx + 16 * i;
+        if (m >= M) continue;
+        const int ikx = m % g.nkx, ikz = m / g.nkx;
+        const double dx = 2 * PI * kx_of(g, ikx) / g.Lx, dz = 2 * PI * ikz / g.Lz;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int y = y0 + ty + 16 * j;
+            if (y >= Ny) continue;
+            double2 p0 = P[0][i][j], p1 = P[1][i][j], p2 = P[2][i][j];
+            double2 q0 = Q[0][i][j], q1 = Q[1][i][j], q2 = Q[2][i][j];
+            if (m == 0) {  // base flow on the kx = kz = 0 mode (nse.cpp:28-36)
+                p0.x += base[0 * Ny + y];
+                q0.x += base[1 * Ny + y];
+                p2.x += base[2 * Ny + y];
+                q2.x += base[3 * Ny + y];
+                p1.x -= Vsuck;
             }
+            double2 o[6];
+            o[0] = p0;
+            o[1] = p1;
+            o[2] = p2;
+            int nout = 3;
+            if (METHOD == NL_ROT) {
+                o[3] = csub(q2, imul(dz, p1));               // dw/dy - dv/dz
+                o[4] = csub(imul(dz, p0), imul(dx, p2));     // du/dz - dw/dx
+                o[5] = csub(imul(dx, p1), q0);               // dv/dx - du/dy
+                nout = 6;
+            } else if (METHOD == NL_CONV || METHOD == NL_SKEW) {
+                o[3] = q0;
+                o[4] = q1;
+                o[5] = q2;
+                nout = 6;
+            }
+            for (int f = 0; f < nout; ++f) yp[((long)f * Ny + y) * M + m] = o[f];
         }
     }
-    for (int j = 0; j < YT; ++j) {
-        const int k = k0 + j;
-        if (k >= Ny) break;
-        for (int i = 0; i < 3; ++i) f[((long)i * Ny + k) * M + m] = A[i][j];
+}
+
+// y values -> Chebyshev coefficients (same outputs as k_yfwd): f_i = Ti G_i + DTi H_i.
+template <int METHOD>
+__global__ void __launch_bounds__(256) k_yfwd_t(Geom g, YMats ym, int have_g, const double2* __restrict__ fo0,
+                                                long sy, const NLMember* __restrict__ tab) {
+    constexpr bool DIVF = (METHOD == NL_DIV || METHOD == NL_SKEW);
+    constexpr int NX = DIVF ? 6 : 3;
+    __shared__ double2 Xs[NX][YB_K][YB_M];
+    __shared__ double As[2][YB_R][YB_K + 1];
+    const double2* __restrict__ fo = fo0 + blockIdx.y * sy;
+    double2* __restrict__ f = reinterpret_cast<double2*>(tab[blockIdx.y].f);
+    const int Ny = g.Ny;
+    const long M = g.M;
+    const int nrt = (Ny + YB_R - 1) / YB_R;
+    const int m0 = (blockIdx.x / nrt) * YB_M, k0o = (blockIdx.x % nrt) * YB_R;
+    const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+    const double beta = (METHOD == NL_SKEW) ? 0.5 : 1.0;
+    double2 Acc[3][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) Acc[c][i][j] = make_double2(0.0, 0.0);
+    for (int y0 = 0; y0 < Ny; y0 += YB_K) {
+        // G_i = g_i + beta (ikx uu_i0 + ikz uu_i2), H_i = beta uu_i1 (uu index -> fo field: 00:3 01:4 02:5 11:6 12:7 22:8)
+        for (int i = threadIdx.x; i < 3 * YB_K * YB_M; i += 256) {
+            const int mm = i % YB_M, kk = (i / YB_M) % YB_K, c = i / (YB_M * YB_K);
+            const int y = y0 + kk, m = m0 + mm;
+            double2 G = make_double2(0.0, 0.0), H = make_double2(0.0, 0.0);
+            if (y < Ny && m < M) {
+                if (have_g) G = fo[((long)c * Ny + y) * M + m];
+                if (DIVF) {
+                    const int ikx = m % g.nkx, ikz = m / g.nkx;
+                    const double dx = 2 * PI * kx_of(g, ikx) / g.Lx, dz = 2 * PI * ikz / g.Lz;
+                    const int uuf[3][3] = {{3, 4, 5}, {4, 6, 7}, {5, 7, 8}};
+                    const double2 x0 = fo[((long)uuf[c][0] * Ny + y) * M + m];
+                    const double2 x1 = fo[((long)uuf[c][1] * Ny + y) * M + m];
+                    const double2 x2 = fo[((long)uuf[c][2] * Ny + y) * M + m];
+                    const double2 d = cadd(imul(dx, x0), imul(dz, x2));
+                    G = make_double2(G.x + beta * d.x, G.y + beta * d.y);
+                    H = make_double2(beta * x1.x, beta * x1.y);
+                }
+            }
+            Xs[c][kk][mm] = G;
+            if (DIVF) Xs[3 + c][kk][mm] = H;
+        }
+        for (int i = threadIdx.x; i < (DIVF ? 2 : 1) * YB_R * YB_K; i += 256) {
+            const int kk = i % YB_K, r = (i / YB_K) % YB_R, a = i / (YB_K * YB_R);
+            const int y = y0 + kk, k = k0o + r;
+            As[a][r][kk] = (k < Ny && y < Ny) ? (a == 0 ? ym.Ti : ym.DTi)[(long)k * Ny + y] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < YB_K; ++kk) {
+            double t[2], d[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                t[j] = As[0][ty + 16 * j][kk];
+                d[j] = DIVF ? As[1][ty + 16 * j][kk] : 0.0;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double2 xg[2], xh[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    xg[i] = Xs[c][kk][tx + 16 * i];
+                    xh[i] = DIVF ? Xs[3 * DIVF + c][kk][tx + 16 * i] : make_double2(0.0, 0.0);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        Acc[c][i][j].x += t[j] * xg[i].x;
+                        Acc[c][i][j].y += t[j] * xg[i].y;
+                        if (DIVF) {
+                            Acc[c][i][j].x += d[j] * xh[i].x;
+                            Acc[c][i][j].y += d[j] * xh[i].y;
+                        }
+                    }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int m = m0 + tx + 16 * i;
+        if (m >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int k = k0o + ty + 16 * j;
+            if (k >= Ny) continue;
+            for (int c = 0; c < 3; ++c) f[((long)c * Ny + k) * M + m] = Acc[c][i][j];
+        }
     }
 }
 
@@ -1078,7 +1195,7 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
                       KTimer* kt) {
     KTimer dummy;
     KTimer& T = kt ? *kt : dummy;
-    const dim3 ygrid((g.M + 63) / 64, (g.Ny + YT - 1) / YT, n);
+    const dim3 ygrid((unsigned)(((g.M + YB_M - 1) / YB_M) * ((g.Ny + YB_R - 1) / YB_R)), n);
     const long sy = nlwork_y_doubles(g) / 2, sx = nlwork_x_doubles(g) / 2;  // per-member strides in double2
     double2* yp = reinterpret_cast<double2*>(w.yp);
     double2* xp = reinterpret_cast<double2*>(w.xp);
@@ -1088,19 +1205,19 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     hipEvent_t ev = T.begin();
     switch (method) {
         case NL_ROT:
-            k_yinv<NL_ROT><<<ygrid, 64, 0, st>>>(g, ym, tab, yp, sy);
+            k_yinv_t<NL_ROT><<<ygrid, 256, 0, st>>>(g, ym, tab, yp, sy);
             nsrc = 6; nxout = 6;
             break;
         case NL_CONV:
-            k_yinv<NL_CONV><<<ygrid, 64, 0, st>>>(g, ym, tab, yp, sy);
+            k_yinv_t<NL_CONV><<<ygrid, 256, 0, st>>>(g, ym, tab, yp, sy);
             nsrc = 6; nxout = 9;
             break;
         case NL_SKEW:
-            k_yinv<NL_SKEW><<<ygrid, 64, 0, st>>>(g, ym, tab, yp, sy);
+            k_yinv_t<NL_SKEW><<<ygrid, 256, 0, st>>>(g, ym, tab, yp, sy);
             nsrc = 6; nxout = 9;
             break;
         default:
-            k_yinv<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, tab, yp, sy);
+            k_yinv_t<NL_DIV><<<ygrid, 256, 0, st>>>(g, ym, tab, yp, sy);
             nsrc = 3; nxout = 3;
     }
     T.end(K_YINV, ev);
@@ -1123,10 +1240,10 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     T.end(K_XFWD, ev);
     ev = T.begin();
     switch (method) {
-        case NL_ROT: k_yfwd<NL_ROT><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, sy, tab); break;
-        case NL_CONV: k_yfwd<NL_CONV><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, sy, tab); break;
-        case NL_SKEW: k_yfwd<NL_SKEW><<<ygrid, 64, 0, st>>>(g, ym, 1, fo, sy, tab); break;
-        default: k_yfwd<NL_DIV><<<ygrid, 64, 0, st>>>(g, ym, want_g, fo, sy, tab);
+        case NL_ROT: k_yfwd_t<NL_ROT><<<ygrid, 256, 0, st>>>(g, ym, 1, fo, sy, tab); break;
+        case NL_CONV: k_yfwd_t<NL_CONV><<<ygrid, 256, 0, st>>>(g, ym, 1, fo, sy, tab); break;
+        case NL_SKEW: k_yfwd_t<NL_SKEW><<<ygrid, 256, 0, st>>>(g, ym, 1, fo, sy, tab); break;
+        default: k_yfwd_t<NL_DIV><<<ygrid, 256, 0, st>>>(g, ym, want_g, fo, sy, tab);
     }
     T.end(K_YFWD, ev);
 }

@@ -344,6 +344,309 @@ __device__ __forceinline__ double whelm_mean(const WV<K>& f, const double* band,
     return mu;
 }
 
+// ---- several right-hand sides per scan ----------------------------------------
+// The Chebyshev-tau operators are real, so the real and imaginary parts of a
+// mode (and u and w, which share the velocity operator) are solved together:
+// one affine scan carries NR constant terms with a shared multiplier, which
+// keeps the arithmetic of each right-hand side exactly that of the one-RHS
+// routines above while the dependent shuffle chains are shared.
+
+template <int K, int NR>
+__device__ __forceinline__ void waff_suffix_n(double (&cc)[NR][K], const double (&aa)[K], int lane) {
+    double Fc[NR][K], Fa[K];
+    double xc[NR], xa = 1.0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) xc[r] = 0.0;
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            xc[r] = cc[r][k] + aa[k] * xc[r];
+            Fc[r][k] = xc[r];
+        }
+        xa = aa[k] * xa;
+        Fa[k] = xa;
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        double yc[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) yc[r] = wdn(xc[r], d);
+        const double ya = wdn(xa, d);
+        if (lane + d < 64) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) xc[r] = xc[r] + xa * yc[r];
+            xa = xa * ya;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        double ec = wdn(xc[r], 1);
+        if (lane == 63) ec = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) cc[r][k] = Fc[r][k] + Fa[k] * ec;
+    }
+}
+
+template <int K, int NR>
+__device__ __forceinline__ void waff_prefix_n(double (&cc)[NR][K], const double (&aa)[K], int lane) {
+    double Gc[NR][K], Ga[K];
+    double xc[NR], xa = 1.0;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) xc[r] = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            xc[r] = cc[r][k] + aa[k] * xc[r];
+            Gc[r][k] = xc[r];
+        }
+        xa = aa[k] * xa;
+        Ga[k] = xa;
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        double yc[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) yc[r] = wup(xc[r], d);
+        const double ya = wup(xa, d);
+        if (lane >= d) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) xc[r] = xc[r] + xa * yc[r];
+            xa = xa * ya;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        double ec = wup(xc[r], 1);
+        if (lane == 0) ec = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) cc[r][k] = Gc[r][k] + Ga[k] * ec;
+    }
+}
+
+// sums over the wave of NR values, identical on every lane (as wsum)
+template <int NR>
+__device__ __forceinline__ void wsum_n(double (&v)[NR], int lane) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        double y[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) y[r] = wdn(v[r], d);
+        if (lane + d < 64) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) v[r] += y[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) v[r] = wbc(v[r], 0);
+}
+
+// whelm_par for NR right-hand sides f[r] with boundary values bc[r]
+template <int K, int NR>
+__device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const double (&band)[K], const double (&up)[K],
+                                            const double (&invd)[K], double lambda, const double (&bc)[NR], int par,
+                                            int M, const WCtx& c, double (&x)[NR][K]) {
+    const int lane = c.lane, Nc = c.Nc;
+    double cc[NR][K], aa[K];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        double fm[K], fp[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            fm[k] = (k > 0) ? f[r][k - 1] : 0.0;
+            fp[k] = (k + 1 < K) ? f[r][k + 1] : 0.0;
+        }
+        const double lastp = wup(f[r][K - 1], 1);
+        const double firstn = wdn(f[r][0], 1);
+        if (lane > 0) fm[0] = lastp;
+        fp[K - 1] = (lane < 63) ? firstn : 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = K * lane + k;
+            const int n = 2 * i + par;
+            double u = 0.0;
+            if (i == 0) {
+                u = bc[r];
+            } else if (i < M) {
+                const int nn = n * n;
+                const double lo = ((double)wh_c(n - 2, Nc)) / (4 * n * (n - 1));
+                const double dg = -((double)wh_beta(n, Nc)) / (2 * (nn - 1));
+                if (i < M - 1) {
+                    const double upv = wh_beta(n + 2, Nc) ? 1.0 / (4 * n * (n + 1)) : 0.0;
+                    u = lo * fm[k] + dg * f[r][k] + upv * fp[k];
+                } else {
+                    u = lo * fm[k] + dg * f[r][k];
+                }
+            }
+            x[r][k] = u;
+            cc[r][k] = (i >= 1 && i < M) ? u : 0.0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = K * lane + k;
+        aa[k] = (i >= 1 && i < M - 1) ? -up[k] : ((i == M - 1 && i >= 1) ? 0.0 : 1.0);
+    }
+    waff_suffix_n<K, NR>(cc, aa, lane);
+    // row 0: x0 = (u0 - sum_{j>=1} band_j x_j) / band_0
+    double part[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        part[r] = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = K * lane + k;
+            if (i >= 1 && i < M) part[r] += band[k] * cc[r][k];
+        }
+    }
+    wsum_n<NR>(part, lane);
+    const double b0 = wbc(band[0], 0);
+    double x0[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) x0[r] = (wbc(x[r][0], 0) - part[r]) / b0;
+    // forward substitution y_i = (x_i - lo_i y_{i-1}) invd_i
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = K * lane + k;
+        const int n = 2 * i + par;
+        if (i == 0) {
+            aa[k] = 0.0;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) cc[r][k] = x0[r];
+        } else if (i < M) {
+            const double lo = -(wh_c(n - 2, Nc) * lambda) / (4 * n * (n - 1));
+            aa[k] = -lo * invd[k];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) cc[r][k] = cc[r][k] * invd[k];
+        } else {
+            aa[k] = 1.0;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) cc[r][k] = 0.0;
+        }
+    }
+    waff_prefix_n<K, NR>(cc, aa, lane);
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int k = 0; k < K; ++k) x[r][k] = (K * lane + k < M) ? cc[r][k] : 0.0;
+}
+
+// whelm for NR profiles sharing one operator; Dirichlet values ua[r], ub[r]
+template <int K, int NR>
+__device__ __forceinline__ void whelm_n(const WV<K> (&f)[NR], const double* band, const double* up, const double* invd,
+                                        double lambda, const double (&ua)[NR], const double (&ub)[NR], const WCtx& c,
+                                        WV<K> (&x)[NR]) {
+    const WV<K> B = wload<K>(band, c), U = wload<K>(up, c), I = wload<K>(invd, c);
+    double fe[NR][K], fo[NR][K], xe[NR][K], xo[NR][K], be[NR], bo[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        be[r] = (ub[r] + ua[r]) / 2;
+        bo[r] = (ub[r] - ua[r]) / 2;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            fe[r][k] = f[r].e[k];
+            fo[r][k] = f[r].o[k];
+        }
+    }
+    whelm_par_n<K, NR>(fe, B.e, U.e, I.e, lambda, be, 0, c.Me, c, xe);
+    whelm_par_n<K, NR>(fo, B.o, U.o, I.o, lambda, bo, 1, c.Mo, c, xo);
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            x[r].e[k] = xe[r][k];
+            x[r].o[k] = xo[r][k];
+        }
+}
+
+// suffix sums of NR arrays at once (as wsuffix)
+template <int K, int NR>
+__device__ __forceinline__ void wsuffix_n(const double (&t)[NR][K], double (&incl)[NR][K], double (&excl)[NR][K],
+                                          int lane) {
+    double loc[NR][K], x[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k) {
+            acc = t[r][k] + acc;
+            loc[r][k] = acc;
+        }
+        x[r] = loc[r][0];
+    }
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        double y[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) y[r] = wdn(x[r], d);
+        if (lane + d < 64) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) x[r] += y[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        double after = wdn(x[r], 1);
+        if (lane == 63) after = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            incl[r][k] = loc[r][k] + after;
+            excl[r][k] = (k + 1 < K) ? loc[r][k + 1] + after : after;
+        }
+    }
+}
+
+// d/dy of NR profiles (as wdiff)
+template <int K, int NR>
+__device__ __forceinline__ void wdiff_n(const WV<K> (&u)[NR], const WCtx& c, WV<K> (&d)[NR]) {
+    const double s = 4.0 / c.L;
+    double to[NR][K], te[NR][K], inc[NR][K], exc[NR][K];
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = K * c.lane + k;
+            to[r][k] = (i < c.Mo) ? s * (2 * i + 1) * u[r].o[k] : 0.0;
+            te[r][k] = (i < c.Me) ? s * (2 * i) * u[r].e[k] : 0.0;
+        }
+    wsuffix_n<K, NR>(to, inc, exc, c.lane);
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int k = 0; k < K; ++k) d[r].e[k] = (K * c.lane + k < c.Me) ? inc[r][k] : 0.0;
+    wsuffix_n<K, NR>(te, inc, exc, c.lane);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) d[r].o[k] = (K * c.lane + k < c.Mo) ? exc[r][k] : 0.0;
+        if (c.lane == 0) d[r].e[0] *= 0.5;
+    }
+}
+
+// values at y=b and y=a of NR profiles (as wends)
+template <int K, int NR>
+__device__ __forceinline__ void wends_n(const WV<K> (&v)[NR], const WCtx& c, double (&at_b)[NR], double (&at_a)[NR]) {
+    double s2[2 * NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        double se = 0.0, so = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            se += v[r].e[k];
+            so += v[r].o[k];
+        }
+        s2[2 * r] = se;
+        s2[2 * r + 1] = so;
+    }
+    wsum_n<2 * NR>(s2, c.lane);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        at_b[r] = s2[2 * r] + s2[2 * r + 1];
+        at_a[r] = s2[2 * r] - s2[2 * r + 1];
+    }
+}
+
 // complex profile <-> LDS row of double2 (re, im) per coefficient n
 template <int K>
 __device__ __forceinline__ void wload_c(const double2* row, const WCtx& c, WV<K>& re, WV<K>& im) {
