@@ -231,22 +231,22 @@ __device__ __forceinline__ CV cview(double* base, const Geom& g, int comp, int m
     return CV{SV{p, S2}, SV{p + 1, S2}};
 }
 
-// Tau solve with one wavefront per live mode (NSE::solve, nse.cpp:479-575;
+// Tau solve with one W-lane segment of a wavefront per live mode (NSE::solve, nse.cpp:479-575;
 // TauSolver::solve, tausolver.cpp:347-450, with solve_P_and_v :193-251).
-constexpr int TAU_WPB = 8;  // waves (modes) per block; 8 complex doubles = one 128-byte row
+constexpr int TAU_WPB = 8;  // modes per block (W lanes each); 8 complex doubles = one 128-byte row
 
 // solve_P_and_v for the real and imaginary parts at once (NR = 2 right-hand
 // sides per scan); per part the same arithmetic as the one-RHS routines
 // (tausolver.cpp:193-251).
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, const double* cm, const double* sc, double kappa2,
                                                   double lambda, double nu, bool taucorr, const WV<K> (&r)[2],
                                                   const WV<K> (&Ry)[2], WV<K> (&P)[2], WV<K> (&v)[2]) {
     const long Ny = c.N;
     const double z2[2] = {0.0, 0.0};
-    whelm_n<K, 2>(r, cm + 0 * Ny, cm + 1 * Ny, cm + 2 * Ny, kappa2, z2, z2, c, P);
+    whelm_n<K, W, 2>(r, cm + 0 * Ny, cm + 1 * Ny, cm + 2 * Ny, kappa2, z2, z2, c, P);
     WV<K> tmp[2];
-    wdiff_n<K, 2>(P, c, tmp);
+    wdiff_n<K, W, 2>(P, c, tmp);
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -254,14 +254,14 @@ __device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, const double* c
             tmp[p].e[k] -= Ry[p].e[k];
             tmp[p].o[k] -= Ry[p].o[k];
         }
-    whelm_n<K, 2>(tmp, cm + 3 * Ny, cm + 4 * Ny, cm + 5 * Ny, lambda, z2, z2, c, v);
+    whelm_n<K, W, 2>(tmp, cm + 3 * Ny, cm + 4 * Ny, cm + 5 * Ny, lambda, z2, z2, c, v);
     // influence-matrix correction (tausolver.cpp:178-191)
     double dvb[2], dva[2];
-    wdiff_n<K, 2>(v, c, tmp);
-    wends_n<K, 2>(tmp, c, dvb, dva);
+    wdiff_n<K, W, 2>(v, c, tmp);
+    wends_n<K, W, 2>(tmp, c, dvb, dva);
     {
-        const WV<K> Pp = wload<K>(cm + 6 * Ny, c), Pm = wload<K>(cm + 7 * Ny, c);
-        const WV<K> vp = wload<K>(cm + 8 * Ny, c), vm = wload<K>(cm + 9 * Ny, c);
+        const WV<K> Pp = wload<K, W>(cm + 6 * Ny, c), Pm = wload<K, W>(cm + 7 * Ny, c);
+        const WV<K> vp = wload<K, W>(cm + 8 * Ny, c), vm = wload<K, W>(cm + 9 * Ny, c);
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
             const double dplus = -sc[0] * dvb[p] - sc[1] * dva[p];
@@ -277,12 +277,12 @@ __device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, const double* c
     }
     if (!taucorr) return;
     const int Nb = c.Nb;
-    const WV<K> P0 = wload<K>(cm + 10 * Ny, c), v0 = wload<K>(cm + 11 * Ny, c);
+    const WV<K> P0 = wload<K, W>(cm + 10 * Ny, c), v0 = wload<K, W>(cm + 11 * Ny, c);
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        double sigma1_Nb = lambda * wtop<K>(v[p], c) - nu * 0.0 - wtop<K>(Ry[p], c);
-        double sigma1_Nb1 = lambda * wtop1<K>(v[p], c) - nu * 0.0 - wtop1<K>(Ry[p], c);
-        sigma1_Nb1 += (4.0 / c.L) * Nb * wtop<K>(P[p], c);
+        double sigma1_Nb = lambda * wtop<K, W>(v[p], c) - nu * 0.0 - wtop<K, W>(Ry[p], c);
+        double sigma1_Nb1 = lambda * wtop1<K, W>(v[p], c) - nu * 0.0 - wtop1<K, W>(Ry[p], c);
+        sigma1_Nb1 += (4.0 / c.L) * Nb * wtop<K, W>(P[p], c);
         const double sigmaNb = sigma1_Nb / (1.0 - sc[5]);
         const double sigmaNb1 = sigma1_Nb1 / (1.0 - sc[4]);
 #pragma unroll
@@ -300,8 +300,8 @@ __device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, const double* c
 // Runge-Kutta combination, dnsalgo.cpp:218-224, 668-673) with full-line loads
 // into LDS as [mode][c][n], each wave solves its mode in registers, and u, v,
 // w, q leave through LDS the same way.
-template <int K>
-__global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const TauMember* __restrict__ tab) {
+template <int K, int W>
+__global__ void __launch_bounds__(W * TAU_WPB) k_tau_solve_wave(Geom g, const TauMember* __restrict__ tab) {
     extern __shared__ double2 lds[];
     const TauMember& me = tab[blockIdx.y];
     const TauSet s = me.set;
@@ -312,9 +312,9 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const T
     double2* __restrict__ q = reinterpret_cast<double2*>(me.q);
     const int nterms = me.nterms;
     const int Ny = g.Ny;
-    const int nthr = 64 * TAU_WPB;
+    const int nthr = W * TAU_WPB;
     const int m0 = blockIdx.x * TAU_WPB;
-    const int w = threadIdx.x >> 6;
+    const int w = threadIdx.x / W;  // mode slot of this lane's segment
     const int m = m0 + w;
     const bool valid = m < g.M;
     for (int idx = threadIdx.x; idx < 3 * Ny * TAU_WPB; idx += nthr) {
@@ -342,7 +342,7 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const T
     c.Mo = (Ny - 1) / 2;
     c.Nb = Ny - 1;
     c.Nc = Ny - 1;
-    c.lane = threadIdx.x & 63;
+    c.lane = threadIdx.x % W;
     c.L = g.b - g.a;
     const int ikx = m % g.nkx, ikz = m / g.nkx;
     const int kx = kx_of(g, ikx), kz = ikz;
@@ -356,11 +356,11 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const T
     const double* sc = s.scal + (long)(valid ? m : 0) * 6;
     WV<K> Rxr, Rxi, Ryr, Ryi, Rzr, Rzi;
     if (valid) {
-        wload_c<K>(lds + (w * 3 + 0) * Ny, c, Rxr, Rxi);
-        wload_c<K>(lds + (w * 3 + 1) * Ny, c, Ryr, Ryi);
-        wload_c<K>(lds + (w * 3 + 2) * Ny, c, Rzr, Rzi);
+        wload_c<K, W>(lds + (w * 3 + 0) * Ny, c, Rxr, Rxi);
+        wload_c<K, W>(lds + (w * 3 + 1) * Ny, c, Ryr, Ryi);
+        wload_c<K, W>(lds + (w * 3 + 2) * Ny, c, Rzr, Rzi);
         if (zero) {  // kx = kz = 0 (nse.cpp:505-527): nu U'', nu W'' and the mean pressure gradient
-            const WV<K> Uyy = wload<K>(zc.nuUbaseyy, c), Wyy = wload<K>(zc.nuWbaseyy, c);
+            const WV<K> Uyy = wload<K, W>(zc.nuUbaseyy, c), Wyy = wload<K, W>(zc.nuWbaseyy, c);
             for (int k = 0; k < K; ++k) {
                 Rxr.e[k] += Uyy.e[k];
                 Rxr.o[k] += Uyy.o[k];
@@ -378,32 +378,32 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const T
     if (valid) {
         WV<K> vr, vi;
         if (zero) {  // solve_P_and_v for kx = kz = 0: P from the pressure Helmholtz, v = 0
-            Pr = whelm<K>(wdiff<K>(Ryr, c), cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
-            Pi = whelm<K>(wdiff<K>(Ryi, c), cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
-            vr = wzero<K>();
-            vi = wzero<K>();
+            Pr = whelm<K, W>(wdiff<K, W>(Ryr, c), cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
+            Pi = whelm<K, W>(wdiff<K, W>(Ryi, c), cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
+            vr = wzero<K, W>();
+            vi = wzero<K, W>();
         } else {  // real and imaginary parts together
             const WV<K> Ry2[2] = {Ryr, Ryi};
             WV<K> rr[2], P2[2], v2[2];
-            wdiff_n<K, 2>(Ry2, c, rr);
+            wdiff_n<K, W, 2>(Ry2, c, rr);
             for (int k = 0; k < K; ++k) {
                 rr[0].e[k] -= kx2pi * Rxi.e[k] + kz2pi * Rzi.e[k];
                 rr[0].o[k] -= kx2pi * Rxi.o[k] + kz2pi * Rzi.o[k];
                 rr[1].e[k] += kx2pi * Rxr.e[k] + kz2pi * Rzr.e[k];
                 rr[1].o[k] += kx2pi * Rxr.o[k] + kz2pi * Rzr.o[k];
             }
-            tau_P_and_v_wave2<K>(c, cm, sc, kappa2, lambda, nu, taucorr != 0, rr, Ry2, P2, v2);
+            tau_P_and_v_wave2<K, W>(c, cm, sc, kappa2, lambda, nu, taucorr != 0, rr, Ry2, P2, v2);
             Pr = P2[0];
             Pi = P2[1];
             vr = v2[0];
             vi = v2[1];
         }
         if (zero) {  // real parts only (nse.cpp:555-563)
-            Pi = wzero<K>();
-            vi = wzero<K>();
+            Pi = wzero<K, W>();
+            vi = wzero<K, W>();
         }
-        wstore_c<K>(lds + (w * 3 + 0) * Ny, c, Pr, Pi);
-        wstore_c<K>(lds + (w * 3 + 1) * Ny, c, vr, vi);
+        wstore_c<K, W>(lds + (w * 3 + 0) * Ny, c, Pr, Pi);
+        wstore_c<K, W>(lds + (w * 3 + 1) * Ny, c, vr, vi);
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < 2 * Ny * TAU_WPB; idx += nthr) {
@@ -433,13 +433,13 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const T
             }
             WV<K> xr, xi;
             if (zero && zc.bulk) {  // bulk-velocity constraint (tausolver.cpp:404-450)
-                const double mu = whelm_mean<K>(fr, vb, vu, vd, lambda, nu, umean, c, xr);
+                const double mu = whelm_mean<K, W>(fr, vb, vu, vd, lambda, nu, umean, c, xr);
                 if (c.lane == 0) *mu_out = mu;
             } else {
-                xr = whelm<K>(fr, vb, vu, vd, lambda, 0.0, 0.0, c);
+                xr = whelm<K, W>(fr, vb, vu, vd, lambda, 0.0, 0.0, c);
             }
-            xi = zero ? wzero<K>() : whelm<K>(fi, vb, vu, vd, lambda, 0.0, 0.0, c);
-            wstore_c<K>(lds + (w * 3 + slot) * Ny, c, xr, xi);
+            xi = zero ? wzero<K, W>() : whelm<K, W>(fi, vb, vu, vd, lambda, 0.0, 0.0, c);
+            wstore_c<K, W>(lds + (w * 3 + slot) * Ny, c, xr, xi);
         };
         if (zero) {
             solve_uw(kx2pi, Rxr, Rxi, zc.umean, zc.dPdx_out, 0);
@@ -457,9 +457,9 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const T
                 f4[3].o[k] = kz2pi * Pr.o[k] - Rzi.o[k];
             }
             const double z4[4] = {0.0, 0.0, 0.0, 0.0};
-            whelm_n<K, 4>(f4, vb, vu, vd, lambda, z4, z4, c, x4);
-            wstore_c<K>(lds + (w * 3 + 0) * Ny, c, x4[0], x4[1]);
-            wstore_c<K>(lds + (w * 3 + 1) * Ny, c, x4[2], x4[3]);
+            whelm_n<K, W, 4>(f4, vb, vu, vd, lambda, z4, z4, c, x4);
+            wstore_c<K, W>(lds + (w * 3 + 0) * Ny, c, x4[0], x4[1]);
+            wstore_c<K, W>(lds + (w * 3 + 1) * Ny, c, x4[2], x4[3]);
         }
     }
     __syncthreads();
@@ -473,13 +473,17 @@ __global__ void __launch_bounds__(64 * TAU_WPB) k_tau_solve_wave(Geom g, const T
 void launch_tau_solve(const Geom& g, int n, const TauMember* tab, hipStream_t st) {
     const dim3 grid((g.M + TAU_WPB - 1) / TAU_WPB, n);
     const size_t lds = (size_t)3 * g.Ny * TAU_WPB * sizeof(double2);
-    const int Me = (g.Ny + 1) / 2;
-    if (Me <= 64)
-        k_tau_solve_wave<1><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
+    const int Me = (g.Ny + 1) / 2;  // even coefficients: at most K per lane of a W-lane segment
+    if (Me <= 16)
+        k_tau_solve_wave<1, 16><<<grid, 16 * TAU_WPB, lds, st>>>(g, tab);
+    else if (Me <= 32)
+        k_tau_solve_wave<1, 32><<<grid, 32 * TAU_WPB, lds, st>>>(g, tab);
+    else if (Me <= 64)
+        k_tau_solve_wave<1, 64><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
     else if (Me <= 128)
-        k_tau_solve_wave<2><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
+        k_tau_solve_wave<2, 64><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
     else
-        k_tau_solve_wave<3><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
+        k_tau_solve_wave<3, 64><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
 }
 
 // ===========================================================================

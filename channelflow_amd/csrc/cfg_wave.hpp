@@ -1,4 +1,6 @@
-// Wavefront-parallel Chebyshev-tau algebra for one (kx, kz) mode per wave64.
+// Wavefront-parallel Chebyshev-tau algebra: one (kx, kz) mode per segment of
+// W lanes of a wave64 (W = 16, 32 or 64; 64/W modes per wave). Every shuffle
+// is width-W, so scans and reductions never cross a segment.
 //
 // A Chebyshev profile of N (odd) coefficients is held split by parity, the
 // way the Chebyshev-tau Helmholtz systems decouple: E[i] = c[2i] (Me = (N+1)/2
@@ -24,24 +26,28 @@ struct WV {
 };
 
 struct WCtx {
-    int N, Me, Mo, Nb, Nc, lane;
+    int N, Me, Mo, Nb, Nc, lane;  // lane: index within the mode's segment
     double L;
 };
 
-__device__ __forceinline__ double wdn(double v, int d) { return __shfl_down(v, d, 64); }
-__device__ __forceinline__ double wup(double v, int d) { return __shfl_up(v, d, 64); }
-__device__ __forceinline__ double wbc(double v, int src) { return __shfl(v, src, 64); }
+template <int W>
+__device__ __forceinline__ double wdn(double v, int d) { return __shfl_down(v, d, W); }
+template <int W>
+__device__ __forceinline__ double wup(double v, int d) { return __shfl_up(v, d, W); }
+template <int W>
+__device__ __forceinline__ double wbc(double v, int src) { return __shfl(v, src, W); }
 
-// sum over the wave, result identical on every lane (tree to lane 0, broadcast)
+// sum over the segment, result identical on every lane (tree to lane 0, broadcast)
+template <int W>
 __device__ __forceinline__ double wsum(double v, int lane) {
-    for (int d = 32; d >= 1; d >>= 1) {
-        const double y = wdn(v, d);
-        if (lane + d < 64) v += y;
+    for (int d = W / 2; d >= 1; d >>= 1) {
+        const double y = wdn<W>(v, d);
+        if (lane + d < W) v += y;
     }
-    return wbc(v, 0);
+    return wbc<W>(v, 0);
 }
 
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ WV<K> wzero() {
     WV<K> v;
     for (int k = 0; k < K; ++k) v.e[k] = v.o[k] = 0.0;
@@ -49,7 +55,7 @@ __device__ __forceinline__ WV<K> wzero() {
 }
 
 // contiguous coefficients c[0..N) (stride 1)
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ WV<K> wload(const double* p, const WCtx& c) {
     WV<K> v;
     for (int k = 0; k < K; ++k) {
@@ -61,7 +67,7 @@ __device__ __forceinline__ WV<K> wload(const double* p, const WCtx& c) {
 }
 
 // strided coefficients c[n] at p[n * s]
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ WV<K> wload_s(const double* p, long s, const WCtx& c) {
     WV<K> v;
     for (int k = 0; k < K; ++k) {
@@ -72,7 +78,7 @@ __device__ __forceinline__ WV<K> wload_s(const double* p, long s, const WCtx& c)
     return v;
 }
 
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ void wstore_s(double* p, long s, const WV<K>& v, const WCtx& c) {
     for (int k = 0; k < K; ++k) {
         const int i = K * c.lane + k;
@@ -85,7 +91,7 @@ __device__ __forceinline__ void wstore_s(double* p, long s, const WV<K>& v, cons
 // Blocked entry i of one parity array, broadcast to all lanes. Each candidate
 // passes through an empty asm operand: without it the select chain is folded
 // into an indexed load, which puts the array in scratch memory.
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ double wpick(const double (&a)[K], int i, const WCtx& c) {
     const int owner = i / K, slot = i - owner * K;
     double x = 0.0;
@@ -94,17 +100,17 @@ __device__ __forceinline__ double wpick(const double (&a)[K], int i, const WCtx&
         asm volatile("" : "+v"(t));
         x = (k == slot) ? t : x;
     }
-    return wbc(x, owner);
+    return wbc<W>(x, owner);
 }
 
 // coefficients N-1 (even, N odd) and N-2 (odd) of v, broadcast to all lanes
-template <int K>
-__device__ __forceinline__ double wtop(const WV<K>& v, const WCtx& c) { return wpick<K>(v.e, c.Me - 1, c); }
-template <int K>
-__device__ __forceinline__ double wtop1(const WV<K>& v, const WCtx& c) { return wpick<K>(v.o, c.Mo - 1, c); }
+template <int K, int W>
+__device__ __forceinline__ double wtop(const WV<K>& v, const WCtx& c) { return wpick<K, W>(v.e, c.Me - 1, c); }
+template <int K, int W>
+__device__ __forceinline__ double wtop1(const WV<K>& v, const WCtx& c) { return wpick<K, W>(v.o, c.Mo - 1, c); }
 
 // a*x + b*y elementwise
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ WV<K> waxpby(double a, const WV<K>& x, double b, const WV<K>& y) {
     WV<K> r;
     for (int k = 0; k < K; ++k) {
@@ -116,7 +122,7 @@ __device__ __forceinline__ WV<K> waxpby(double a, const WV<K>& x, double b, cons
 
 // suffix sums of t over the blocked index: out[i] = sum_{j >= i} t[j]
 // (inclusive) and excl[i] = sum_{j > i} t[j]
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ void wsuffix(const double (&t)[K], double (&incl)[K], double (&excl)[K], int lane) {
     double loc[K];
     double acc = 0.0;
@@ -125,12 +131,12 @@ __device__ __forceinline__ void wsuffix(const double (&t)[K], double (&incl)[K],
         loc[k] = acc;
     }
     double x = loc[0];
-    for (int d = 1; d < 64; d <<= 1) {
-        const double y = wdn(x, d);
-        if (lane + d < 64) x += y;
+    for (int d = 1; d < W; d <<= 1) {
+        const double y = wdn<W>(x, d);
+        if (lane + d < W) x += y;
     }
-    double after = wdn(x, 1);
-    if (lane == 63) after = 0.0;
+    double after = wdn<W>(x, 1);
+    if (lane == W - 1) after = 0.0;
     for (int k = 0; k < K; ++k) {
         incl[k] = loc[k] + after;
         excl[k] = (k + 1 < K) ? loc[k + 1] + after : after;
@@ -139,7 +145,7 @@ __device__ __forceinline__ void wsuffix(const double (&t)[K], double (&incl)[K],
 
 // d/dy of Chebyshev coefficients on an interval of length L:
 // du[2i] = sum_{j>=i} (4/L)(2j+1) c[2j+1],  du[2i+1] = sum_{j>i} (4/L)(2j) c[2j], du[0] halved
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ WV<K> wdiff(const WV<K>& u, const WCtx& c) {
     const double s = 4.0 / c.L;
     double to[K], te[K], inc[K], exc[K];
@@ -149,24 +155,24 @@ __device__ __forceinline__ WV<K> wdiff(const WV<K>& u, const WCtx& c) {
         te[k] = (i < c.Me) ? s * (2 * i) * u.e[k] : 0.0;
     }
     WV<K> d;
-    wsuffix<K>(to, inc, exc, c.lane);
+    wsuffix<K, W>(to, inc, exc, c.lane);
     for (int k = 0; k < K; ++k) d.e[k] = (K * c.lane + k < c.Me) ? inc[k] : 0.0;
-    wsuffix<K>(te, inc, exc, c.lane);
+    wsuffix<K, W>(te, inc, exc, c.lane);
     for (int k = 0; k < K; ++k) d.o[k] = (K * c.lane + k < c.Mo) ? exc[k] : 0.0;
     if (c.lane == 0) d.e[0] *= 0.5;
     return d;
 }
 
 // values at y=b (sum of coefficients) and y=a (alternating sum)
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ void wends(const WV<K>& v, const WCtx& c, double& at_b, double& at_a) {
     double se = 0.0, so = 0.0;
     for (int k = 0; k < K; ++k) {
         se += v.e[k];
         so += v.o[k];
     }
-    se = wsum(se, c.lane);
-    so = wsum(so, c.lane);
+    se = wsum<W>(se, c.lane);
+    so = wsum<W>(so, c.lane);
     at_b = se + so;
     at_a = se - so;
 }
@@ -174,7 +180,7 @@ __device__ __forceinline__ void wends(const WV<K>& v, const WCtx& c, double& at_
 // Affine suffix scan: x_i = cc_i + aa_i x_{i+1}; maps past the chain end are
 // the identity (cc = 0, aa = 1) and the last real map is constant (aa = 0).
 // On return cc[k] holds x_i.
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ void waff_suffix(double (&cc)[K], const double (&aa)[K], int lane) {
     double Fc[K], Fa[K];
     double xc = 0.0, xa = 1.0;
@@ -184,20 +190,20 @@ __device__ __forceinline__ void waff_suffix(double (&cc)[K], const double (&aa)[
         Fc[k] = xc;
         Fa[k] = xa;
     }
-    for (int d = 1; d < 64; d <<= 1) {
-        const double yc = wdn(xc, d), ya = wdn(xa, d);
-        if (lane + d < 64) {
+    for (int d = 1; d < W; d <<= 1) {
+        const double yc = wdn<W>(xc, d), ya = wdn<W>(xa, d);
+        if (lane + d < W) {
             xc = xc + xa * yc;
             xa = xa * ya;
         }
     }
-    double ec = wdn(xc, 1);
-    if (lane == 63) ec = 0.0;
+    double ec = wdn<W>(xc, 1);
+    if (lane == W - 1) ec = 0.0;
     for (int k = 0; k < K; ++k) cc[k] = Fc[k] + Fa[k] * ec;
 }
 
 // Affine prefix scan: y_i = cc_i + aa_i y_{i-1}; the first map is constant.
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ void waff_prefix(double (&cc)[K], const double (&aa)[K], int lane) {
     double Gc[K], Ga[K];
     double xc = 0.0, xa = 1.0;
@@ -207,14 +213,14 @@ __device__ __forceinline__ void waff_prefix(double (&cc)[K], const double (&aa)[
         Gc[k] = xc;
         Ga[k] = xa;
     }
-    for (int d = 1; d < 64; d <<= 1) {
-        const double yc = wup(xc, d), ya = wup(xa, d);
+    for (int d = 1; d < W; d <<= 1) {
+        const double yc = wup<W>(xc, d), ya = wup<W>(xa, d);
         if (lane >= d) {
             xc = xc + xa * yc;
             xa = xa * ya;
         }
     }
-    double ec = wup(xc, 1);
+    double ec = wup<W>(xc, 1);
     if (lane == 0) ec = 0.0;
     for (int k = 0; k < K; ++k) cc[k] = Gc[k] + Ga[k] * ec;
 }
@@ -225,7 +231,7 @@ __device__ __forceinline__ int wh_beta(int n, int Nc) { return (n > Nc - 2) ? 0 
 // One parity system of nu u'' - lambda u = f with u[0] set to bc:
 // x = A^{-1} (B f) using the UL factors band/up/invd of this parity
 // (helmholtz.cpp:79-95 with bandedtridiag.cpp:232-277).
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ void whelm_par(const double (&f)[K], const double (&band)[K], const double (&up)[K],
                                           const double (&invd)[K], double lambda, double bc, int par, int M,
                                           const WCtx& c, double (&x)[K]) {
@@ -237,10 +243,10 @@ __device__ __forceinline__ void whelm_par(const double (&f)[K], const double (&b
         fp[k] = (k + 1 < K) ? f[k + 1] : 0.0;
     }
     {
-        const double lastp = wup(f[K - 1], 1);
-        const double firstn = wdn(f[0], 1);
+        const double lastp = wup<W>(f[K - 1], 1);
+        const double firstn = wdn<W>(f[0], 1);
         if (lane > 0) fm[0] = lastp;
-        if (lane < 63) fp[K - 1] = firstn;
+        if (lane < W - 1) fp[K - 1] = firstn;
         else fp[K - 1] = 0.0;
     }
     // u = B f (rows 1..M-1), u[0] = bc
@@ -275,15 +281,15 @@ __device__ __forceinline__ void whelm_par(const double (&f)[K], const double (&b
             aa[k] = 1.0;
         }
     }
-    waff_suffix<K>(cc, aa, lane);
+    waff_suffix<K, W>(cc, aa, lane);
     // row 0: x0 = (u0 - sum_{j>=1} band_j x_j) / band_0
     double part = 0.0;
     for (int k = 0; k < K; ++k) {
         const int i = K * lane + k;
         if (i >= 1 && i < M) part += band[k] * cc[k];
     }
-    const double s = wsum(part, lane);
-    const double u0 = wbc(x[0], 0), b0 = wbc(band[0], 0);
+    const double s = wsum<W>(part, lane);
+    const double u0 = wbc<W>(x[0], 0), b0 = wbc<W>(band[0], 0);
     const double x0 = (u0 - s) / b0;
     // forward substitution y_i = (x_i - lo_i y_{i-1}) invd_i
     for (int k = 0; k < K; ++k) {
@@ -301,46 +307,46 @@ __device__ __forceinline__ void whelm_par(const double (&f)[K], const double (&b
             aa[k] = 1.0;
         }
     }
-    waff_prefix<K>(cc, aa, lane);
+    waff_prefix<K, W>(cc, aa, lane);
     for (int k = 0; k < K; ++k) x[k] = (K * lane + k < M) ? cc[k] : 0.0;
 }
 
 // Helmholtz solve of a full profile with Dirichlet values ua (y=a), ub (y=b);
 // fac points at the packed factors band[N], up[N], invd[N] (contiguous).
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ WV<K> whelm(const WV<K>& f, const double* band, const double* up, const double* invd,
                                        double lambda, double ua, double ub, const WCtx& c) {
-    const WV<K> B = wload<K>(band, c), U = wload<K>(up, c), I = wload<K>(invd, c);
+    const WV<K> B = wload<K, W>(band, c), U = wload<K, W>(up, c), I = wload<K, W>(invd, c);
     WV<K> x;
-    whelm_par<K>(f.e, B.e, U.e, I.e, lambda, (ub + ua) / 2, 0, c.Me, c, x.e);
-    whelm_par<K>(f.o, B.o, U.o, I.o, lambda, (ub - ua) / 2, 1, c.Mo, c, x.o);
+    whelm_par<K, W>(f.e, B.e, U.e, I.e, lambda, (ub + ua) / 2, 0, c.Me, c, x.e);
+    whelm_par<K, W>(f.o, B.o, U.o, I.o, lambda, (ub - ua) / 2, 1, c.Mo, c, x.o);
     return x;
 }
 
 // mean value over [a,b] (chebyshev.cpp:505-512): c0 - sum_{n even >= 2} c_n/(n^2-1)
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ double wmean(const WV<K>& v, const WCtx& c) {
     double part = 0.0;
     for (int k = 0; k < K; ++k) {
         const int i = K * c.lane + k;
         if (i >= 1 && i < c.Me) part += v.e[k] / (double)(4 * i * i - 1);
     }
-    return wbc(v.e[0], 0) - wsum(part, c.lane);
+    return wbc<W>(v.e[0], 0) - wsum<W>(part, c.lane);
 }
 
 // Helmholtz solve with unknown constant forcing mu and mean(u) = umean
 // (helmholtz.cpp:158-213); returns mu.
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ double whelm_mean(const WV<K>& f, const double* band, const double* up, const double* invd,
                                              double lambda, double nu, double umean, const WCtx& c, WV<K>& u) {
-    const double uamean = wmean<K>(whelm<K>(f, band, up, invd, lambda, 0.0, 0.0, c), c);
-    WV<K> r = wzero<K>();
+    const double uamean = wmean<K, W>(whelm<K, W>(f, band, up, invd, lambda, 0.0, 0.0, c), c);
+    WV<K> r = wzero<K, W>();
     if (c.lane == 0) r.e[0] = nu;
-    const double ucmean = wmean<K>(whelm<K>(r, band, up, invd, lambda, 0.0, 0.0, c), c);
+    const double ucmean = wmean<K, W>(whelm<K, W>(r, band, up, invd, lambda, 0.0, 0.0, c), c);
     const double mu = nu * (umean - uamean) / ucmean;
     r = f;
     if (c.lane == 0) r.e[0] += mu;
-    u = whelm<K>(r, band, up, invd, lambda, 0.0, 0.0, c);
+    u = whelm<K, W>(r, band, up, invd, lambda, 0.0, 0.0, c);
     return mu;
 }
 
@@ -351,7 +357,7 @@ __device__ __forceinline__ double whelm_mean(const WV<K>& f, const double* band,
 // keeps the arithmetic of each right-hand side exactly that of the one-RHS
 // routines above while the dependent shuffle chains are shared.
 
-template <int K, int NR>
+template <int K, int W, int NR>
 __device__ __forceinline__ void waff_suffix_n(double (&cc)[NR][K], const double (&aa)[K], int lane) {
     double Fc[NR][K], Fa[K];
     double xc[NR], xa = 1.0;
@@ -368,12 +374,12 @@ __device__ __forceinline__ void waff_suffix_n(double (&cc)[NR][K], const double 
         Fa[k] = xa;
     }
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
+    for (int d = 1; d < W; d <<= 1) {
         double yc[NR];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) yc[r] = wdn(xc[r], d);
-        const double ya = wdn(xa, d);
-        if (lane + d < 64) {
+        for (int r = 0; r < NR; ++r) yc[r] = wdn<W>(xc[r], d);
+        const double ya = wdn<W>(xa, d);
+        if (lane + d < W) {
 #pragma unroll
             for (int r = 0; r < NR; ++r) xc[r] = xc[r] + xa * yc[r];
             xa = xa * ya;
@@ -381,14 +387,14 @@ __device__ __forceinline__ void waff_suffix_n(double (&cc)[NR][K], const double 
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        double ec = wdn(xc[r], 1);
-        if (lane == 63) ec = 0.0;
+        double ec = wdn<W>(xc[r], 1);
+        if (lane == W - 1) ec = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) cc[r][k] = Fc[r][k] + Fa[k] * ec;
     }
 }
 
-template <int K, int NR>
+template <int K, int W, int NR>
 __device__ __forceinline__ void waff_prefix_n(double (&cc)[NR][K], const double (&aa)[K], int lane) {
     double Gc[NR][K], Ga[K];
     double xc[NR], xa = 1.0;
@@ -405,11 +411,11 @@ __device__ __forceinline__ void waff_prefix_n(double (&cc)[NR][K], const double 
         Ga[k] = xa;
     }
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
+    for (int d = 1; d < W; d <<= 1) {
         double yc[NR];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) yc[r] = wup(xc[r], d);
-        const double ya = wup(xa, d);
+        for (int r = 0; r < NR; ++r) yc[r] = wup<W>(xc[r], d);
+        const double ya = wup<W>(xa, d);
         if (lane >= d) {
 #pragma unroll
             for (int r = 0; r < NR; ++r) xc[r] = xc[r] + xa * yc[r];
@@ -418,7 +424,7 @@ __device__ __forceinline__ void waff_prefix_n(double (&cc)[NR][K], const double 
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        double ec = wup(xc[r], 1);
+        double ec = wup<W>(xc[r], 1);
         if (lane == 0) ec = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) cc[r][k] = Gc[r][k] + Ga[k] * ec;
@@ -426,24 +432,24 @@ __device__ __forceinline__ void waff_prefix_n(double (&cc)[NR][K], const double 
 }
 
 // sums over the wave of NR values, identical on every lane (as wsum)
-template <int NR>
+template <int W, int NR>
 __device__ __forceinline__ void wsum_n(double (&v)[NR], int lane) {
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
+    for (int d = W / 2; d >= 1; d >>= 1) {
         double y[NR];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) y[r] = wdn(v[r], d);
-        if (lane + d < 64) {
+        for (int r = 0; r < NR; ++r) y[r] = wdn<W>(v[r], d);
+        if (lane + d < W) {
 #pragma unroll
             for (int r = 0; r < NR; ++r) v[r] += y[r];
         }
     }
 #pragma unroll
-    for (int r = 0; r < NR; ++r) v[r] = wbc(v[r], 0);
+    for (int r = 0; r < NR; ++r) v[r] = wbc<W>(v[r], 0);
 }
 
 // whelm_par for NR right-hand sides f[r] with boundary values bc[r]
-template <int K, int NR>
+template <int K, int W, int NR>
 __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const double (&band)[K], const double (&up)[K],
                                             const double (&invd)[K], double lambda, const double (&bc)[NR], int par,
                                             int M, const WCtx& c, double (&x)[NR][K]) {
@@ -457,10 +463,10 @@ __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const doub
             fm[k] = (k > 0) ? f[r][k - 1] : 0.0;
             fp[k] = (k + 1 < K) ? f[r][k + 1] : 0.0;
         }
-        const double lastp = wup(f[r][K - 1], 1);
-        const double firstn = wdn(f[r][0], 1);
+        const double lastp = wup<W>(f[r][K - 1], 1);
+        const double firstn = wdn<W>(f[r][0], 1);
         if (lane > 0) fm[0] = lastp;
-        fp[K - 1] = (lane < 63) ? firstn : 0.0;
+        fp[K - 1] = (lane < W - 1) ? firstn : 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = K * lane + k;
@@ -488,7 +494,7 @@ __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const doub
         const int i = K * lane + k;
         aa[k] = (i >= 1 && i < M - 1) ? -up[k] : ((i == M - 1 && i >= 1) ? 0.0 : 1.0);
     }
-    waff_suffix_n<K, NR>(cc, aa, lane);
+    waff_suffix_n<K, W, NR>(cc, aa, lane);
     // row 0: x0 = (u0 - sum_{j>=1} band_j x_j) / band_0
     double part[NR];
 #pragma unroll
@@ -500,11 +506,11 @@ __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const doub
             if (i >= 1 && i < M) part[r] += band[k] * cc[r][k];
         }
     }
-    wsum_n<NR>(part, lane);
-    const double b0 = wbc(band[0], 0);
+    wsum_n<W, NR>(part, lane);
+    const double b0 = wbc<W>(band[0], 0);
     double x0[NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) x0[r] = (wbc(x[r][0], 0) - part[r]) / b0;
+    for (int r = 0; r < NR; ++r) x0[r] = (wbc<W>(x[r][0], 0) - part[r]) / b0;
     // forward substitution y_i = (x_i - lo_i y_{i-1}) invd_i
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -525,7 +531,7 @@ __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const doub
             for (int r = 0; r < NR; ++r) cc[r][k] = 0.0;
         }
     }
-    waff_prefix_n<K, NR>(cc, aa, lane);
+    waff_prefix_n<K, W, NR>(cc, aa, lane);
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
@@ -533,11 +539,11 @@ __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const doub
 }
 
 // whelm for NR profiles sharing one operator; Dirichlet values ua[r], ub[r]
-template <int K, int NR>
+template <int K, int W, int NR>
 __device__ __forceinline__ void whelm_n(const WV<K> (&f)[NR], const double* band, const double* up, const double* invd,
                                         double lambda, const double (&ua)[NR], const double (&ub)[NR], const WCtx& c,
                                         WV<K> (&x)[NR]) {
-    const WV<K> B = wload<K>(band, c), U = wload<K>(up, c), I = wload<K>(invd, c);
+    const WV<K> B = wload<K, W>(band, c), U = wload<K, W>(up, c), I = wload<K, W>(invd, c);
     double fe[NR][K], fo[NR][K], xe[NR][K], xo[NR][K], be[NR], bo[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -549,8 +555,8 @@ __device__ __forceinline__ void whelm_n(const WV<K> (&f)[NR], const double* band
             fo[r][k] = f[r].o[k];
         }
     }
-    whelm_par_n<K, NR>(fe, B.e, U.e, I.e, lambda, be, 0, c.Me, c, xe);
-    whelm_par_n<K, NR>(fo, B.o, U.o, I.o, lambda, bo, 1, c.Mo, c, xo);
+    whelm_par_n<K, W, NR>(fe, B.e, U.e, I.e, lambda, be, 0, c.Me, c, xe);
+    whelm_par_n<K, W, NR>(fo, B.o, U.o, I.o, lambda, bo, 1, c.Mo, c, xo);
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
@@ -561,7 +567,7 @@ __device__ __forceinline__ void whelm_n(const WV<K> (&f)[NR], const double* band
 }
 
 // suffix sums of NR arrays at once (as wsuffix)
-template <int K, int NR>
+template <int K, int W, int NR>
 __device__ __forceinline__ void wsuffix_n(const double (&t)[NR][K], double (&incl)[NR][K], double (&excl)[NR][K],
                                           int lane) {
     double loc[NR][K], x[NR];
@@ -576,19 +582,19 @@ __device__ __forceinline__ void wsuffix_n(const double (&t)[NR][K], double (&inc
         x[r] = loc[r][0];
     }
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
+    for (int d = 1; d < W; d <<= 1) {
         double y[NR];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) y[r] = wdn(x[r], d);
-        if (lane + d < 64) {
+        for (int r = 0; r < NR; ++r) y[r] = wdn<W>(x[r], d);
+        if (lane + d < W) {
 #pragma unroll
             for (int r = 0; r < NR; ++r) x[r] += y[r];
         }
     }
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        double after = wdn(x[r], 1);
-        if (lane == 63) after = 0.0;
+        double after = wdn<W>(x[r], 1);
+        if (lane == W - 1) after = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             incl[r][k] = loc[r][k] + after;
@@ -598,7 +604,7 @@ __device__ __forceinline__ void wsuffix_n(const double (&t)[NR][K], double (&inc
 }
 
 // d/dy of NR profiles (as wdiff)
-template <int K, int NR>
+template <int K, int W, int NR>
 __device__ __forceinline__ void wdiff_n(const WV<K> (&u)[NR], const WCtx& c, WV<K> (&d)[NR]) {
     const double s = 4.0 / c.L;
     double to[NR][K], te[NR][K], inc[NR][K], exc[NR][K];
@@ -610,12 +616,12 @@ __device__ __forceinline__ void wdiff_n(const WV<K> (&u)[NR], const WCtx& c, WV<
             to[r][k] = (i < c.Mo) ? s * (2 * i + 1) * u[r].o[k] : 0.0;
             te[r][k] = (i < c.Me) ? s * (2 * i) * u[r].e[k] : 0.0;
         }
-    wsuffix_n<K, NR>(to, inc, exc, c.lane);
+    wsuffix_n<K, W, NR>(to, inc, exc, c.lane);
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
         for (int k = 0; k < K; ++k) d[r].e[k] = (K * c.lane + k < c.Me) ? inc[r][k] : 0.0;
-    wsuffix_n<K, NR>(te, inc, exc, c.lane);
+    wsuffix_n<K, W, NR>(te, inc, exc, c.lane);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
 #pragma unroll
@@ -625,7 +631,7 @@ __device__ __forceinline__ void wdiff_n(const WV<K> (&u)[NR], const WCtx& c, WV<
 }
 
 // values at y=b and y=a of NR profiles (as wends)
-template <int K, int NR>
+template <int K, int W, int NR>
 __device__ __forceinline__ void wends_n(const WV<K> (&v)[NR], const WCtx& c, double (&at_b)[NR], double (&at_a)[NR]) {
     double s2[2 * NR];
 #pragma unroll
@@ -639,7 +645,7 @@ __device__ __forceinline__ void wends_n(const WV<K> (&v)[NR], const WCtx& c, dou
         s2[2 * r] = se;
         s2[2 * r + 1] = so;
     }
-    wsum_n<2 * NR>(s2, c.lane);
+    wsum_n<W, 2 * NR>(s2, c.lane);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         at_b[r] = s2[2 * r] + s2[2 * r + 1];
@@ -648,7 +654,7 @@ __device__ __forceinline__ void wends_n(const WV<K> (&v)[NR], const WCtx& c, dou
 }
 
 // complex profile <-> LDS row of double2 (re, im) per coefficient n
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ void wload_c(const double2* row, const WCtx& c, WV<K>& re, WV<K>& im) {
     for (int k = 0; k < K; ++k) {
         const int i = K * c.lane + k;
@@ -661,7 +667,7 @@ __device__ __forceinline__ void wload_c(const double2* row, const WCtx& c, WV<K>
     }
 }
 
-template <int K>
+template <int K, int W>
 __device__ __forceinline__ void wstore_c(double2* row, const WCtx& c, const WV<K>& re, const WV<K>& im) {
     for (int k = 0; k < K; ++k) {
         const int i = K * c.lane + k;
