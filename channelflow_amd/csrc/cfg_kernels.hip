@@ -803,80 +803,97 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, const
         zr2c_pair<NZ>(c, zo, 5, 2, -1, -1);
         return;
     }
-    // u, v, w -> phys 0..2
-    zc2r_pair<NZ>(c, xp, 0, false, 0, 1, false, 1);
-    zc2r_pair<NZ>(c, xp, 2, false, 2, -1, false, -1);
-    if (METHOD == NL_CONV || METHOD == NL_SKEW) {
-        // f_i = sum_j (s u_j) du_i/dx_j accumulated over j = x, y, z
-        // (diffops.cpp:3192-3199, 3626-3633); phys 3..5 = f, 6..8 = du_i/dx_j
-        const double s = (METHOD == NL_SKEW) ? 0.5 : 1.0;
-        double *u0 = physf(c, 0), *u1 = physf(c, 1), *u2 = physf(c, 2);
-        double *f0 = physf(c, 3), *f1 = physf(c, 4), *f2 = physf(c, 5);
-        double *t0 = physf(c, 6), *t1 = physf(c, 7), *t2 = physf(c, 8);
-        for (int j = 0; j < 3; ++j) {
-            if (j == 0) {  // d/dx from xp 6..8
-                zc2r_pair<NZ>(c, xp, 6, false, 6, 7, false, 7);
-                zc2r_pair<NZ>(c, xp, 8, false, 8, -1, false, -1);
-            } else if (j == 1) {  // d/dy from xp 3..5
-                zc2r_pair<NZ>(c, xp, 3, false, 6, 4, false, 7);
-                zc2r_pair<NZ>(c, xp, 5, false, 8, -1, false, -1);
-            } else {  // d/dz of u v w
-                zc2r_pair<NZ>(c, xp, 0, true, 6, 1, true, 7);
-                zc2r_pair<NZ>(c, xp, 2, true, 8, -1, false, -1);
-            }
-            const double* uj = (j == 0) ? u0 : (j == 1) ? u1 : u2;
-            for (int w = threadIdx.x; w < npts; w += blockDim.x) {
-                const double su = (METHOD == NL_SKEW) ? s * uj[w] : uj[w];
-                if (j == 0) {
-                    f0[w] = 0.0 + su * t0[w];
-                    f1[w] = 0.0 + su * t1[w];
-                    f2[w] = 0.0 + su * t2[w];
-                } else {
-                    f0[w] += su * t0[w];
-                    f1[w] += su * t1[w];
-                    f2[w] += su * t2[w];
-                }
-            }
-            __syncthreads();
-        }
-        if (rotation != 0.0) {
-            for (int w = threadIdx.x; w < npts; w += blockDim.x) {
-                f0[w] -= rotation * u1[w];
-                f1[w] += rotation * u0[w];
-            }
-            __syncthreads();
-        }
-        zr2c_pair<NZ>(c, zo, 3, 0, 4, 1);
-        zr2c_pair<NZ>(c, zo, 5, 2, -1, -1);
-    } else if (want_g) {  // DIV with rotation: g = rotation terms only
-        double *u0 = physf(c, 0), *u1 = physf(c, 1);
-        double *f0 = physf(c, 3), *f1 = physf(c, 4), *f2 = physf(c, 5);
+    double *u0 = physf(c, 0), *u1 = physf(c, 1), *u2 = physf(c, 2);
+    double *pa = physf(c, 3), *pb = physf(c, 4);
+    if constexpr (METHOD == NL_CONV || METHOD == NL_SKEW) {
+        // f_i = sum_j (s u_j) du_i/dx_j with the terms of each f_i added in the
+        // order j = x, y, z (diffops.cpp:3192-3199, 3626-3633). Component by
+        // component through phys 3, 4 (inputs) and 5, 6 (f); the inputs are
+        // paired so that every two-for-one c2r is full.
+        constexpr bool SK = (METHOD == NL_SKEW);
+        const double s = SK ? 0.5 : 1.0;
+        double *fa = physf(c, 5), *fb = physf(c, 6);
+        auto su = [&](const double* uj, int w) { return SK ? s * uj[w] : uj[w]; };
+        zc2r_pair<NZ>(c, xp, 0, false, 0, 1, false, 1);  // u, v
+        zc2r_pair<NZ>(c, xp, 2, false, 2, 6, false, 3);  // w, du/dx
+        for (int w = threadIdx.x; w < npts; w += blockDim.x) fa[w] = 0.0 + su(u0, w) * pa[w];
+        zc2r_pair<NZ>(c, xp, 3, false, 3, 0, true, 4);   // du/dy, du/dz
         for (int w = threadIdx.x; w < npts; w += blockDim.x) {
-            f0[w] = -(rotation * u1[w]);
-            f1[w] = rotation * u0[w];
-            f2[w] = 0.0;
+            fa[w] += su(u1, w) * pa[w];
+            fa[w] += su(u2, w) * pb[w];
+            if (rotation != 0.0) fa[w] -= rotation * u1[w];  // nse.cpp:61-80
+            if (SK) pb[w] = u0[w] * u0[w];
         }
         __syncthreads();
-        zr2c_pair<NZ>(c, zo, 3, 0, 4, 1);
-        zr2c_pair<NZ>(c, zo, 5, 2, -1, -1);
-    }
-    if (METHOD == NL_DIV || METHOD == NL_SKEW) {  // outer(u, u) (diffops.cpp:2336-2388)
-        double *u0 = physf(c, 0), *u1 = physf(c, 1), *u2 = physf(c, 2);
-        double *p3 = physf(c, 3), *p4 = physf(c, 4), *p5 = physf(c, 5);
-        double *p6 = physf(c, 6), *p7 = physf(c, 7), *p8 = physf(c, 8);
+        zr2c_pair<NZ>(c, zo, 5, 0, SK ? 4 : -1, SK ? 3 : -1);  // f0 (and uu00)
+        zc2r_pair<NZ>(c, xp, 7, false, 3, 4, false, 4);  // dv/dx, dv/dy
         for (int w = threadIdx.x; w < npts; w += blockDim.x) {
-            const double a0 = u0[w], a1 = u1[w], a2 = u2[w];
-            p3[w] = a0 * a0;
-            p4[w] = a0 * a1;
-            p5[w] = a0 * a2;
-            p6[w] = a1 * a1;
-            p7[w] = a1 * a2;
-            p8[w] = a2 * a2;
+            fa[w] = 0.0 + su(u0, w) * pa[w];
+            fa[w] += su(u1, w) * pb[w];
+        }
+        zc2r_pair<NZ>(c, xp, 1, true, 3, 8, false, 4);   // dv/dz, dw/dx
+        for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+            fa[w] += su(u2, w) * pa[w];
+            if (rotation != 0.0) fa[w] += rotation * u0[w];
+            fb[w] = 0.0 + su(u0, w) * pb[w];
+        }
+        zc2r_pair<NZ>(c, xp, 5, false, 3, 2, true, 4);   // dw/dy, dw/dz
+        for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+            fb[w] += su(u1, w) * pa[w];
+            fb[w] += su(u2, w) * pb[w];
+        }
+        __syncthreads();
+        zr2c_pair<NZ>(c, zo, 5, 1, 6, 2);  // f1, f2
+        if (SK) {  // remaining entries of outer(u, u) (diffops.cpp:2336-2388)
+            for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+                pa[w] = u0[w] * u1[w];
+                pb[w] = u0[w] * u2[w];
+            }
+            __syncthreads();
+            zr2c_pair<NZ>(c, zo, 3, 4, 4, 5);
+            for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+                pa[w] = u1[w] * u1[w];
+                pb[w] = u1[w] * u2[w];
+            }
+            __syncthreads();
+            zr2c_pair<NZ>(c, zo, 3, 6, 4, 7);
+            for (int w = threadIdx.x; w < npts; w += blockDim.x) pa[w] = u2[w] * u2[w];
+            __syncthreads();
+            zr2c_pair<NZ>(c, zo, 3, 8, -1, -1);
+        }
+    } else {
+        zc2r_pair<NZ>(c, xp, 0, false, 0, 1, false, 1);
+        zc2r_pair<NZ>(c, xp, 2, false, 2, -1, false, -1);
+        if (want_g) {  // DIV with rotation: g = rotation terms only
+            for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+                pa[w] = -(rotation * u1[w]);
+                pb[w] = rotation * u0[w];
+            }
+            __syncthreads();
+            zr2c_pair<NZ>(c, zo, 3, 0, 4, 1);
+            for (int w = threadIdx.x; w < npts; w += blockDim.x) pa[w] = 0.0;
+            __syncthreads();
+            zr2c_pair<NZ>(c, zo, 3, 2, -1, -1);
+        }
+        // outer(u, u) (diffops.cpp:2336-2388), two entries per r2c
+        for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+            pa[w] = u0[w] * u0[w];
+            pb[w] = u0[w] * u1[w];
         }
         __syncthreads();
         zr2c_pair<NZ>(c, zo, 3, 3, 4, 4);
-        zr2c_pair<NZ>(c, zo, 5, 5, 6, 6);
-        zr2c_pair<NZ>(c, zo, 7, 7, 8, 8);
+        for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+            pa[w] = u0[w] * u2[w];
+            pb[w] = u1[w] * u1[w];
+        }
+        __syncthreads();
+        zr2c_pair<NZ>(c, zo, 3, 5, 4, 6);
+        for (int w = threadIdx.x; w < npts; w += blockDim.x) {
+            pa[w] = u1[w] * u2[w];
+            pb[w] = u2[w] * u2[w];
+        }
+        __syncthreads();
+        zr2c_pair<NZ>(c, zo, 3, 7, 4, 8);
     }
 }
 
@@ -1096,6 +1113,9 @@ __global__ void __launch_bounds__(256) k_yfwd_t(Geom g, YMats ym, int have_g, co
 static size_t g_lds_budget = 64 * 1024;
 void set_lds_budget(size_t bytes) { g_lds_budget = bytes; }
 
+// real physical lines per (y, x) line the z-pass keeps in LDS
+static constexpr int zpass_nphys(int method) { return method == NL_ROT ? 6 : (method == NL_DIV ? 5 : 7); }
+
 // largest z-pass lines per block that fit the LDS budget (power of two <= 32)
 static int zpass_xb_max(const Geom& g, int nphys) {
     const long per = 32L * lpitch(g.Nz) + 8L * g.Nz * nphys;  // two complex work lines + nphys real lines
@@ -1161,7 +1181,7 @@ static void launch_xfft(bool inverse, const Geom& g, const FFTPlan& px, int nsrc
 template <int METHOD, int NZ>
 static void launch_z_t(const Geom& g, const FFTPlan& pz, int n, const NLMember* tab, int want_g, const double2* xp,
                        double2* zo, long sx, const NLTune& tn, hipStream_t st) {
-    const int nphys = (METHOD == NL_ROT) ? 6 : 9;
+    const int nphys = zpass_nphys(METHOD);
     const int XB = zpass_xb(g, nphys, tn);
     const size_t lds = (size_t)g.Nz * sizeof(double2) +
                        (size_t)XB * (2 * lpitch(g.Nz) * sizeof(double2) + (size_t)g.Nz * nphys * sizeof(double));
@@ -1265,7 +1285,7 @@ NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     double2* xp = reinterpret_cast<double2*>(w.xp);
     double2* zo = reinterpret_cast<double2*>(w.zo);
     const int want_g = (method != NL_DIV) || rotating;
-    const int nphys = (method == NL_ROT) ? 6 : 9;
+    const int nphys = zpass_nphys(method);
     const int nxout = (method == NL_ROT) ? 6 : (method == NL_DIV ? 3 : 9);
     const int nsrc = (method == NL_DIV) ? 3 : 6;
     hipEvent_t a, b;
