@@ -1109,6 +1109,332 @@ __global__ void __launch_bounds__(256) k_yfwd_t(Geom g, YMats ym, int have_g, co
     }
 }
 
+// ---- y transforms on the FP64 matrix cores ----------------------------------
+// v_mfma_f64_16x16x4f64 (one wave: D = A B + C, A 16x4, B 4x16, D 16x16).
+// Operand layout, lane l: A[l % 16][l / 16], B[l / 16][l % 16]; result
+// register r of lane l is D[4 r + l / 16][l % 16].
+//
+// A block is three waves per transform matrix; wave c handles velocity
+// component c of YW_MODES = 8 consecutive modes, i.e. 16 real columns (real,
+// imaginary part of each mode), over all output rows. Each lane's B operand
+// streams from HBM straight into registers, one chunk ahead; the chunk of the
+// transform matrices, shared by the waves, is staged in LDS.
+//
+// Both transforms use the mirror symmetry of the Chebyshev points,
+// y_{Ny-1-j} = -y_j: T[Ny-1-j][k] = (-1)^k T[j][k], and the derivative flips
+// the parity. Only the upper half of the points (j < (Ny+1)/2) enters the
+// products, split by the parity of k, which halves the matrix work.
+typedef double mfma4d __attribute__((ext_vector_type(4)));
+constexpr int YW_MODES = 8;
+
+__device__ __forceinline__ mfma4d mfma16x16x4(double a, double b, mfma4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Waves per SIMD the register allocation of a wave-column kernel must allow:
+// the accumulators take 16 NRT registers, the rest is operand staging (without
+// a bound the scheduler hoists a whole chunk's LDS reads).
+constexpr int yw_waves(int nrt) { return nrt <= 4 ? 4 : (nrt <= 6 ? 3 : 2); }
+// The forward transform also holds sums and differences of mirrored values of
+// the prefetched chunk.
+constexpr int yw_waves_fwd(int nrt) { return nrt <= 1 ? 4 : (nrt <= 3 ? 3 : 2); }
+
+// Chebyshev -> y values (outputs as k_yinv_t): p = T u and, unless the form is
+// the divergence one, q = T D u = du/dy values. Waves 0-2 compute p of
+// components 0-2, waves 3-5 q; the two halves read the same coefficient
+// columns (the second read is served by the caches). For an upper point
+// j < Ny/2, E = sum over even k, O = sum over odd k: p_j = E + O,
+// p_{Ny-1-j} = E - O, q_j = E + O, q_{Ny-1-j} = O - E. The middle point of an
+// odd Ny is a dot product per lane, summed over the lane groups. NRT: row
+// tiles of 16 upper points.
+template <int METHOD, int NRT>
+__global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((amdgpu_waves_per_eu(yw_waves(NRT))))
+    k_yinv_w(Geom g, YMats ym, const NLMember* __restrict__ tab, double2* __restrict__ yp0, long sy) {
+    constexpr int NMAT = (METHOD == NL_DIV) ? 1 : 2, NT = 192 * NMAT;
+    constexpr int KC = 16, AP = KC + 1, EP = 17;  // chunk columns: 8 even k, then 8 odd k
+    constexpr int RS = NRT * 16 + 1;              // LDS rows per matrix: tiles, middle point
+    constexpr int SA = NMAT * RS * AP, SE = NMAT * 3 * 2 * 16 * EP;
+    __shared__ double sm[SA > SE ? SA : SE];
+    const NLMember& me = tab[blockIdx.y];
+    double2* __restrict__ yp = yp0 + blockIdx.y * sy;
+    const int Ny = g.Ny, nm = Ny / 2, mid = (Ny & 1) ? nm : -1;
+    const long M = g.M;
+    const int nrt = (nm + 15) / 16;
+    const int w = threadIdx.x >> 6, c = w % 3, am = w / 3, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+    const long m0 = (long)blockIdx.x * YW_MODES;
+    const long mcol = m0 + (lr >> 1);
+    const bool mok = mcol < M;
+    const double* __restrict__ ucol = me.u + 2 * ((long)c * Ny * M + mcol) + (l & 1);
+    const double* __restrict__ arow = sm + (am * RS + lr) * AP + lk;
+    const double* __restrict__ amid = sm + (am * RS + NRT * 16) * AP + lk;
+    mfma4d ae[NRT], ao[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt) ae[rt] = ao[rt] = mfma4d{0.0, 0.0, 0.0, 0.0};
+    double dmid = 0.0;
+    double bn[4];  // [parity * 2 + step]: k = k0 + 2 (4 step + lane / 16) + parity
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int k = k0 + 2 * (4 * (i & 1) + lk) + (i >> 1);
+            bn[i] = (mok && k < Ny) ? ucol[2L * k * M] : 0.0;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < Ny; k0 += KC) {
+        double b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[i] = bn[i];
+        __syncthreads();  // the previous chunk's matrix reads are done
+        for (int i = threadIdx.x; i < NMAT * RS * KC; i += NT) {
+            const int kk = i % KC, slot = (i / KC) % RS, a = i / (KC * RS);
+            const int k = k0 + kk, row = slot < NRT * 16 ? slot : mid;
+            const bool ok = (slot < NRT * 16 ? slot < nm : mid >= 0) && k < Ny;
+            sm[(a * RS + slot) * AP + (kk & 1) * 8 + (kk >> 1)] = ok ? (a == 0 ? ym.T : ym.TD)[(long)row * Ny + k] : 0.0;
+        }
+        if (k0 + KC < Ny) fetch(k0 + KC);
+        __syncthreads();
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+            // unconditional: staged rows and columns past the grid are zero,
+            // and a conditional update keeps the tiles from staying in place
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt) {
+                ae[rt] = mfma16x16x4(arow[rt * 16 * AP + 4 * st], b[st], ae[rt]);
+                ao[rt] = mfma16x16x4(arow[rt * 16 * AP + 8 + 4 * st], b[2 + st], ao[rt]);
+            }
+            dmid = fma(amid[4 * st], b[st], fma(amid[8 + 4 * st], b[2 + st], dmid));
+            __builtin_amdgcn_sched_barrier(0);  // keep each step's operand reads next to its MFMAs
+        }
+    }
+    const int pm = threadIdx.x & 7;
+    const long m = m0 + pm;
+    const int ikx = (int)(m % g.nkx), ikz = (int)(m / g.nkx);
+    const double dx = 2 * PI * kx_of(g, ikx) / g.Lx, dz = 2 * PI * ikz / g.Lz;
+    // outputs of one point from the exchange rows: p of component cc at e + cc * ps, q at e + (3 + cc) * ps
+    auto emit = [&](int y, const double* e, int ps) {
+        double2 p[3], q[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            p[cc] = make_double2(e[cc * ps], e[cc * ps + 1]);
+            q[cc] = NMAT == 2 ? make_double2(e[(3 + cc) * ps], e[(3 + cc) * ps + 1]) : make_double2(0.0, 0.0);
+        }
+        if (m == 0) {  // base flow on the kx = kz = 0 mode (nse.cpp:28-36)
+            const double* __restrict__ base = me.base;
+            p[0].x += base[0 * Ny + y];
+            q[0].x += base[1 * Ny + y];
+            p[2].x += base[2 * Ny + y];
+            q[2].x += base[3 * Ny + y];
+            p[1].x -= me.Vsuck;
+        }
+        double2 o[6];
+        o[0] = p[0];
+        o[1] = p[1];
+        o[2] = p[2];
+        int nout = 3;
+        if (METHOD == NL_ROT) {
+            o[3] = csub(q[2], imul(dz, p[1]));            // dw/dy - dv/dz
+            o[4] = csub(imul(dz, p[0]), imul(dx, p[2]));  // du/dz - dw/dx
+            o[5] = csub(imul(dx, p[1]), q[0]);            // dv/dx - du/dy
+            nout = 6;
+        } else if (METHOD == NL_CONV || METHOD == NL_SKEW) {
+            o[3] = q[0];
+            o[4] = q[1];
+            o[5] = q[2];
+            nout = 6;
+        }
+        for (int f = 0; f < nout; ++f) yp[((long)f * Ny + y) * M + m] = o[f];
+    };
+    // row tiles: the components and both halves meet in LDS
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt) {
+        if (rt >= nrt) continue;  // uniform
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double e = ae[rt][r], o = ao[rt][r];
+            sm[(((am * 3 + c) * 2 + 0) * 16 + 4 * r + lk) * EP + lr] = e + o;
+            sm[(((am * 3 + c) * 2 + 1) * 16 + 4 * r + lk) * EP + lr] = am == 0 ? e - o : o - e;
+        }
+        __syncthreads();
+        for (int pt = threadIdx.x; pt < 256; pt += NT) {
+            const int half = pt >> 7, pr = (pt >> 3) & 15;
+            const int yt = rt * 16 + pr;
+            if (yt >= nm || m >= M) continue;
+            emit(half ? Ny - 1 - yt : yt, sm + (half * 16 + pr) * EP + 2 * pm, 2 * 16 * EP);
+        }
+    }
+    if (mid < 0) return;  // uniform
+    dmid += __shfl_xor(dmid, 16);
+    dmid += __shfl_xor(dmid, 32);
+    __syncthreads();
+    if (lk == 0) sm[(am * 3 + c) * EP + lr] = dmid;
+    __syncthreads();
+    if (threadIdx.x < YW_MODES && m < M) emit(mid, sm + 2 * pm, EP);
+}
+
+// y values -> Chebyshev coefficients (outputs as k_yfwd_t): f_c = Ti G_c +
+// DTi H_c, with G_c = g_c + beta (i kx uu_c0 + i kz uu_c2) and H_c = beta uu_c1
+// for the divergence forms (beta 1/2 in the skew-symmetric one), G_c = g_c
+// otherwise. With S, D the sum and difference of the values at an upper point
+// and its mirror (the middle point of an odd Ny counted once):
+// f_{2i} = Ti S_G + DTi D_H, f_{2i+1} = Ti D_G + DTi S_H over the upper points.
+// The MFMA tiles hold i < Ny/2; the last coefficient of an odd Ny (k = Ny-1)
+// is a dot product per lane, summed over the lane groups. NRT: row tiles of 16
+// coefficients of each parity.
+template <int METHOD, int NRT>
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_waves_fwd(NRT)))) k_yfwd_w(Geom g, YMats ym, int have_g, const double2* __restrict__ fo0,
+                                                long sy, const NLMember* __restrict__ tab) {
+    constexpr bool DIVF = (METHOD == NL_DIV || METHOD == NL_SKEW);
+    constexpr int NMAT = DIVF ? 2 : 1;
+    constexpr int KC = 8, AP = KC + 1, NS = KC / 4;
+    constexpr int RS = 2 * NRT * 16 + 1;  // LDS rows per matrix: even, odd coefficients, last one
+    __shared__ double sm[NMAT * RS * AP];
+    const double* __restrict__ fo = reinterpret_cast<const double*>(fo0 + blockIdx.y * sy);
+    double* __restrict__ f = tab[blockIdx.y].f;
+    const int Ny = g.Ny, nh = (Ny + 1) / 2, nm = Ny / 2;
+    const bool odd = Ny & 1;
+    const long M = g.M;
+    const int c = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4, part = l & 1;
+    const long mcol = (long)blockIdx.x * YW_MODES + (lr >> 1);
+    const bool mok = mcol < M;
+    const double beta = (METHOD == NL_SKEW) ? 0.5 : 1.0;
+    double sdx = 0, sdz = 0;  // i k multiplication of the other part, with its sign
+    if (DIVF) {
+        const int ikx = (int)(mcol % g.nkx), ikz = (int)(mcol / g.nkx);
+        const double sg = part ? beta : -beta;
+        sdx = sg * 2 * PI * kx_of(g, ikx) / g.Lx;
+        sdz = sg * 2 * PI * ikz / g.Lz;
+    }
+    // field of uu_cj in fo: 00:3 01:4 02:5 11:6 12:7 22:8
+    const int f0 = c == 0 ? 3 : (c == 1 ? 4 : 5), f1 = c == 0 ? 4 : (c == 1 ? 6 : 7), f2 = c == 0 ? 5 : (c == 1 ? 7 : 8);
+    const long rowst = 2 * M;
+    const double* __restrict__ pg = fo + 2 * ((long)c * Ny * M + mcol) + part;
+    const double* __restrict__ p0 = fo + 2 * ((long)f0 * Ny * M + mcol) + (1 - part);
+    const double* __restrict__ p1 = fo + 2 * ((long)f1 * Ny * M + mcol) + part;
+    const double* __restrict__ p2 = fo + 2 * ((long)f2 * Ny * M + mcol) + (1 - part);
+    mfma4d ae[NRT], ao[NRT];
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt) ae[rt] = ao[rt] = mfma4d{0.0, 0.0, 0.0, 0.0};
+    double dlast = 0.0;
+    auto gval = [&](int y) {
+        double G = have_g ? pg[y * rowst] : 0.0;
+        if (DIVF) G += sdx * p0[y * rowst] + sdz * p2[y * rowst];
+        return G;
+    };
+    double sgn[NS], dgn[NS], shn[NS], dhn[NS];
+    auto fetch = [&](int y0) {
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int y = y0 + 4 * s + lk, ymr = Ny - 1 - y;
+            double G = 0, Gm = 0, H = 0, Hm = 0;
+            if (mok && y < nh) {
+                G = gval(y);
+                if (DIVF) H = beta * p1[y * rowst];
+                if (ymr != y) {
+                    Gm = gval(ymr);
+                    if (DIVF) Hm = beta * p1[ymr * rowst];
+                }
+            }
+            sgn[s] = G + Gm;
+            dgn[s] = G - Gm;
+            shn[s] = H + Hm;
+            dhn[s] = H - Hm;
+        }
+    };
+    fetch(0);
+    for (int y0 = 0; y0 < nh; y0 += KC) {
+        double sg[NS], dg[NS], sh[NS], dh[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            sg[s] = sgn[s];
+            dg[s] = dgn[s];
+            sh[s] = shn[s];
+            dh[s] = dhn[s];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < NMAT * RS * KC; i += 192) {
+            const int kk = i % KC, slot = (i / KC) % RS, a = i / (KC * RS);
+            const int par = slot / (NRT * 16), r = slot % (NRT * 16), y = y0 + kk;  // par 2: the last coefficient
+            const int k = par < 2 ? 2 * r + par : Ny - 1;
+            const bool ok = (par < 2 ? r < nm : odd) && y < nh;
+            sm[(a * RS + slot) * AP + kk] = ok ? (a == 0 ? ym.Ti : ym.DTi)[(long)k * Ny + y] : 0.0;
+        }
+        if (y0 + KC < nh) fetch(y0 + KC);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            // unconditional: staged rows and columns past the grid are zero,
+            // and a conditional update keeps the tiles from staying in place
+#pragma unroll
+            for (int rt = 0; rt < NRT; ++rt) {
+                const double* A = sm + (rt * 16 + lr) * AP + 4 * s + lk;
+                ae[rt] = mfma16x16x4(A[0], sg[s], ae[rt]);
+                ao[rt] = mfma16x16x4(A[NRT * 16 * AP], dg[s], ao[rt]);
+                if (DIVF) {
+                    ae[rt] = mfma16x16x4(A[RS * AP], dh[s], ae[rt]);
+                    ao[rt] = mfma16x16x4(A[(RS + NRT * 16) * AP], sh[s], ao[rt]);
+                }
+            }
+            const double* A = sm + 2 * NRT * 16 * AP + 4 * s + lk;
+            dlast = fma(A[0], sg[s], dlast);
+            if (DIVF) dlast = fma(A[RS * AP], dh[s], dlast);
+            __builtin_amdgcn_sched_barrier(0);  // keep each step's operand reads next to its MFMAs
+        }
+    }
+    dlast += __shfl_xor(dlast, 16);
+    dlast += __shfl_xor(dlast, 32);
+    if (!mok) return;
+    double* __restrict__ fc = f + 2 * ((long)c * Ny * M + mcol) + part;
+#pragma unroll
+    for (int rt = 0; rt < NRT; ++rt) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = rt * 16 + 4 * r + lk;
+            if (i < nm) {
+                fc[2 * i * rowst] = ae[rt][r];
+                fc[(2 * i + 1) * rowst] = ao[rt][r];
+            }
+        }
+    }
+    if (odd && lk == 0) fc[(Ny - 1) * rowst] = dlast;
+}
+
+// Row-tile count classes with a wave-column kernel; above the last one the
+// vector tiles run.
+template <int METHOD>
+static void launch_yinv(const Geom& g, const YMats& ym, int n, const NLMember* tab, double2* yp, long sy,
+                        hipStream_t st) {
+    const int nrt = (g.Ny / 2 + 15) / 16, nt = METHOD == NL_DIV ? 192 : 384;
+    const dim3 grid((unsigned)((g.M + YW_MODES - 1) / YW_MODES), n);
+    if (nrt <= 1) k_yinv_w<METHOD, 1><<<grid, nt, 0, st>>>(g, ym, tab, yp, sy);
+    else if (nrt <= 2) k_yinv_w<METHOD, 2><<<grid, nt, 0, st>>>(g, ym, tab, yp, sy);
+    else if (nrt <= 3) k_yinv_w<METHOD, 3><<<grid, nt, 0, st>>>(g, ym, tab, yp, sy);
+    else if (nrt <= 4) k_yinv_w<METHOD, 4><<<grid, nt, 0, st>>>(g, ym, tab, yp, sy);
+    else if (nrt <= 6) k_yinv_w<METHOD, 6><<<grid, nt, 0, st>>>(g, ym, tab, yp, sy);
+    else if (nrt <= 8) k_yinv_w<METHOD, 8><<<grid, nt, 0, st>>>(g, ym, tab, yp, sy);
+    else {
+        const dim3 tg((unsigned)(((g.M + YB_M - 1) / YB_M) * ((g.Ny + YB_R - 1) / YB_R)), n);
+        k_yinv_t<METHOD><<<tg, 256, 0, st>>>(g, ym, tab, yp, sy);
+    }
+}
+
+template <int METHOD>
+static void launch_yfwd(const Geom& g, const YMats& ym, int n, int have_g, const double2* fo, long sy,
+                        const NLMember* tab, hipStream_t st) {
+    const int nrt = (g.Ny / 2 + 15) / 16;
+    const dim3 grid((unsigned)((g.M + YW_MODES - 1) / YW_MODES), n);
+    if (nrt <= 1) k_yfwd_w<METHOD, 1><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
+    else if (nrt <= 2) k_yfwd_w<METHOD, 2><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
+    else if (nrt <= 3) k_yfwd_w<METHOD, 3><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
+    else if (nrt <= 4) k_yfwd_w<METHOD, 4><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
+    else if (nrt <= 6) k_yfwd_w<METHOD, 6><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
+    else if (nrt <= 8) k_yfwd_w<METHOD, 8><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
+    else {
+        const dim3 tg((unsigned)(((g.M + YB_M - 1) / YB_M) * ((g.Ny + YB_R - 1) / YB_R)), n);
+        k_yfwd_t<METHOD><<<tg, 256, 0, st>>>(g, ym, have_g, fo, sy, tab);
+    }
+}
+
 // LDS budget per block (set from the device's opt-in limit at start-up)
 static size_t g_lds_budget = 64 * 1024;
 void set_lds_budget(size_t bytes) { g_lds_budget = bytes; }
@@ -1219,7 +1545,6 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
                       KTimer* kt) {
     KTimer dummy;
     KTimer& T = kt ? *kt : dummy;
-    const dim3 ygrid((unsigned)(((g.M + YB_M - 1) / YB_M) * ((g.Ny + YB_R - 1) / YB_R)), n);
     const long sy = nlwork_y_doubles(g) / 2, sx = nlwork_x_doubles(g) / 2;  // per-member strides in double2
     double2* yp = reinterpret_cast<double2*>(w.yp);
     double2* xp = reinterpret_cast<double2*>(w.xp);
@@ -1229,19 +1554,19 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     hipEvent_t ev = T.begin();
     switch (method) {
         case NL_ROT:
-            k_yinv_t<NL_ROT><<<ygrid, 256, 0, st>>>(g, ym, tab, yp, sy);
+            launch_yinv<NL_ROT>(g, ym, n, tab, yp, sy, st);
             nsrc = 6; nxout = 6;
             break;
         case NL_CONV:
-            k_yinv_t<NL_CONV><<<ygrid, 256, 0, st>>>(g, ym, tab, yp, sy);
+            launch_yinv<NL_CONV>(g, ym, n, tab, yp, sy, st);
             nsrc = 6; nxout = 9;
             break;
         case NL_SKEW:
-            k_yinv_t<NL_SKEW><<<ygrid, 256, 0, st>>>(g, ym, tab, yp, sy);
+            launch_yinv<NL_SKEW>(g, ym, n, tab, yp, sy, st);
             nsrc = 6; nxout = 9;
             break;
         default:
-            k_yinv_t<NL_DIV><<<ygrid, 256, 0, st>>>(g, ym, tab, yp, sy);
+            launch_yinv<NL_DIV>(g, ym, n, tab, yp, sy, st);
             nsrc = 3; nxout = 3;
     }
     T.end(K_YINV, ev);
@@ -1264,10 +1589,10 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     T.end(K_XFWD, ev);
     ev = T.begin();
     switch (method) {
-        case NL_ROT: k_yfwd_t<NL_ROT><<<ygrid, 256, 0, st>>>(g, ym, 1, fo, sy, tab); break;
-        case NL_CONV: k_yfwd_t<NL_CONV><<<ygrid, 256, 0, st>>>(g, ym, 1, fo, sy, tab); break;
-        case NL_SKEW: k_yfwd_t<NL_SKEW><<<ygrid, 256, 0, st>>>(g, ym, 1, fo, sy, tab); break;
-        default: k_yfwd_t<NL_DIV><<<ygrid, 256, 0, st>>>(g, ym, want_g, fo, sy, tab);
+        case NL_ROT: launch_yfwd<NL_ROT>(g, ym, n, 1, fo, sy, tab, st); break;
+        case NL_CONV: launch_yfwd<NL_CONV>(g, ym, n, 1, fo, sy, tab, st); break;
+        case NL_SKEW: launch_yfwd<NL_SKEW>(g, ym, n, 1, fo, sy, tab, st); break;
+        default: launch_yfwd<NL_DIV>(g, ym, n, want_g, fo, sy, tab, st);
     }
     T.end(K_YFWD, ev);
 }
