@@ -233,7 +233,6 @@ __device__ __forceinline__ CV cview(double* base, const Geom& g, int comp, int m
 
 // Tau solve with one W-lane segment of a wavefront per live mode (NSE::solve, nse.cpp:479-575;
 // TauSolver::solve, tausolver.cpp:347-450, with solve_P_and_v :193-251).
-constexpr int TAU_WPB = 8;  // modes per block (W lanes each); 8 complex doubles = one 128-byte row
 
 // solve_P_and_v for the real and imaginary parts at once (NR = 2 right-hand
 // sides per scan); per part the same arithmetic as the one-RHS routines
@@ -295,13 +294,13 @@ __device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, const double* c
     }
 }
 
-// Block of TAU_WPB consecutive modes of member blockIdx.y: the right-hand side
-// rows [c][n][m0..m0+WPB) are formed from the member's terms (the multistep /
+// Block of MPB consecutive modes of member blockIdx.y: the right-hand side
+// rows [c][n][m0..m0+MPB) are formed from the member's terms (the multistep /
 // Runge-Kutta combination, dnsalgo.cpp:218-224, 668-673) with full-line loads
 // into LDS as [mode][c][n], each wave solves its mode in registers, and u, v,
 // w, q leave through LDS the same way.
-template <int K, int W>
-__global__ void __launch_bounds__(W * TAU_WPB) k_tau_solve_wave(Geom g, const TauMember* __restrict__ tab) {
+template <int K, int W, int MPB>
+__global__ void __launch_bounds__(W * MPB) k_tau_solve_wave(Geom g, const TauMember* __restrict__ tab) {
     extern __shared__ double2 lds[];
     const TauMember& me = tab[blockIdx.y];
     const TauSet s = me.set;
@@ -312,13 +311,13 @@ __global__ void __launch_bounds__(W * TAU_WPB) k_tau_solve_wave(Geom g, const Ta
     double2* __restrict__ q = reinterpret_cast<double2*>(me.q);
     const int nterms = me.nterms;
     const int Ny = g.Ny;
-    const int nthr = W * TAU_WPB;
-    const int m0 = blockIdx.x * TAU_WPB;
+    const int nthr = W * MPB;
+    const int m0 = blockIdx.x * MPB;
     const int w = threadIdx.x / W;  // mode slot of this lane's segment
     const int m = m0 + w;
     const bool valid = m < g.M;
-    for (int idx = threadIdx.x; idx < 3 * Ny * TAU_WPB; idx += nthr) {
-        const int ww = idx % TAU_WPB, rest = idx / TAU_WPB, n = rest % Ny, comp = rest / Ny;
+    for (int idx = threadIdx.x; idx < 3 * Ny * MPB; idx += nthr) {
+        const int ww = idx % MPB, rest = idx / MPB, n = rest % Ny, comp = rest / Ny;
         const int mm = m0 + ww;
         double2 acc = make_double2(0.0, 0.0);
         if (mm < g.M) {
@@ -406,8 +405,8 @@ __global__ void __launch_bounds__(W * TAU_WPB) k_tau_solve_wave(Geom g, const Ta
         wstore_c<K, W>(lds + (w * 3 + 1) * Ny, c, vr, vi);
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < 2 * Ny * TAU_WPB; idx += nthr) {
-        const int ww = idx % TAU_WPB, rest = idx / TAU_WPB, n = rest % Ny, slot = rest / Ny;
+    for (int idx = threadIdx.x; idx < 2 * Ny * MPB; idx += nthr) {
+        const int ww = idx % MPB, rest = idx / MPB, n = rest % Ny, slot = rest / Ny;
         const int mm = m0 + ww;
         if (mm < g.M) {
             const double2 v = lds[(ww * 3 + slot) * Ny + n];
@@ -463,27 +462,31 @@ __global__ void __launch_bounds__(W * TAU_WPB) k_tau_solve_wave(Geom g, const Ta
         }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < 2 * Ny * TAU_WPB; idx += nthr) {
-        const int ww = idx % TAU_WPB, rest = idx / TAU_WPB, n = rest % Ny, slot = rest / Ny;
+    for (int idx = threadIdx.x; idx < 2 * Ny * MPB; idx += nthr) {
+        const int ww = idx % MPB, rest = idx / MPB, n = rest % Ny, slot = rest / Ny;
         const int mm = m0 + ww;
         if (mm < g.M) u[((long)(slot == 0 ? 0 : 2) * Ny + n) * g.M + mm] = lds[(ww * 3 + slot) * Ny + n];
     }
 }
 
 void launch_tau_solve(const Geom& g, int n, const TauMember* tab, hipStream_t st) {
-    const dim3 grid((g.M + TAU_WPB - 1) / TAU_WPB, n);
-    const size_t lds = (size_t)3 * g.Ny * TAU_WPB * sizeof(double2);
     const int Me = (g.Ny + 1) / 2;  // even coefficients: at most K per lane of a W-lane segment
-    if (Me <= 16)
-        k_tau_solve_wave<1, 16><<<grid, 16 * TAU_WPB, lds, st>>>(g, tab);
-    else if (Me <= 32)
-        k_tau_solve_wave<1, 32><<<grid, 32 * TAU_WPB, lds, st>>>(g, tab);
-    else if (Me <= 64)
-        k_tau_solve_wave<1, 64><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
-    else if (Me <= 128)
-        k_tau_solve_wave<2, 64><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
-    else
-        k_tau_solve_wave<3, 64><<<grid, 64 * TAU_WPB, lds, st>>>(g, tab);
+    // 256-thread blocks: several per CU, so that one block's HBM staging overlaps
+    // another's scans
+    auto go = [&](auto kern, int w) {
+        const int mpb = 256 / w;
+        const dim3 grid((unsigned)((g.M + mpb - 1) / mpb), n);
+        const size_t lds = (size_t)3 * g.Ny * mpb * sizeof(double2);
+        kern<<<grid, 256, lds, st>>>(g, tab);
+    };
+    // one coefficient per lane where it fits (4 waves per SIMD); above 64 even
+    // coefficients two modes per wave at three per lane beat one at two
+    if (Me <= 16) go(k_tau_solve_wave<1, 16, 16>, 16);
+    else if (Me <= 32) go(k_tau_solve_wave<1, 32, 8>, 32);
+    else if (Me <= 64) go(k_tau_solve_wave<1, 64, 4>, 64);
+    else if (Me <= 96) go(k_tau_solve_wave<3, 32, 8>, 32);
+    else if (Me <= 128) go(k_tau_solve_wave<2, 64, 4>, 64);
+    else go(k_tau_solve_wave<3, 64, 4>, 64);
 }
 
 // ===========================================================================
