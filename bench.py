@@ -16,10 +16,14 @@ The same JSON line also carries
   served      a serving run: Poisson arrivals of Couette (Re 300-500) and
               Poiseuille (Re_bulk 1500-3000) requests on grids 32^3 ... 128^3,
               horizons 100-2000 steps, admitted by HBM budget, with p50/p99
-              completion latency and aggregate grid-point*steps/s;
+              completion latency and aggregate grid-point*steps/s, beside the
+              same requests run one at a time on the plain path;
   single_c2   the plain 192x129x192 Poiseuille channel (rotational, SBDF3,
               dt = 0.005) and its per-kernel times;
-  roofline    the dominant kernel of the headline workload (HIP events);
+  graphs      hipGraph captures and replays inside the timed region;
+  roofline    the dominant kernel of the headline workload (HIP events over
+              a profiled pass of the same ensemble right after the timed
+              region; replay is off while events are recorded);
   cpu_baseline the oracle (oracle/cpu_ref.cpp, one core) on a bounded sample.
 """
 import argparse
@@ -123,15 +127,22 @@ def kernel_bytes(grid, method):
 # ---------------------------------------------------------------------------
 # runs
 # ---------------------------------------------------------------------------
-def run_batched(cfa, members, steps, warmup):
+def run_batched(cfa, members, steps, warmup, prof_steps):
+    """Timed region: `steps` steps of the whole ensemble with each class's
+    repeating ticks replayed as hipGraphs and no events between launches.
+    Then `prof_steps` more steps with HIP events around every launch (graphs
+    off while events are on) give the per-kernel durations."""
     batch = cfa.Batch()
     for mem in members:
         batch.add(flow_fields(cfa, mem["grid"], member_field(mem)), cfa.DNSFlags(**mem["flags"]))
-    batch.advance(warmup)                 # SMRK2 start + warm-up SBDF3 steps, untimed
-    batch.profile(True)                   # HIP events around every launch on each class stream
+    batch.advance(warmup)                 # SMRK2 start + warm-up SBDF3 steps (and graph captures), untimed
+    c0, r0 = batch.graph_stats()
     t0 = time.perf_counter()
     batch.advance(steps)                  # returns after every stream has drained
     wall = time.perf_counter() - t0
+    c1, r1 = batch.graph_stats()
+    batch.profile(True)
+    batch.advance(prof_steps)
     kt = batch.kernel_times()
     batch.profile(False)
     for mid in range(0, len(members), max(1, len(members) // 8)):
@@ -139,7 +150,8 @@ def run_batched(cfa, members, steps, warmup):
         if not np.all(np.isfinite(u)):
             raise SystemExit(f"non-finite state of member {mid}")
     pts = sum(np.prod(m["grid"]) for m in members)
-    return dict(value=pts * steps / wall, wall=wall, kt=kt, pts=pts, classes=batch.size()[1])
+    return dict(value=pts * steps / wall, wall=wall, kt=kt, pts=pts, classes=batch.size()[1],
+                graphs={"captured_in_timed_region": c1 - c0, "replayed_in_timed_region": r1 - r0})
 
 
 def run_plain(cfa, members, steps, warmup):
@@ -191,6 +203,7 @@ def run_served(cfa, n, rate, seed, budget):
     lat = {}
     t0 = time.perf_counter()
     max_running = 0
+    next_report = 5.0
     while len(lat) < n:
         now = time.perf_counter() - t0
         while submitted < n and arrivals[submitted] <= now:
@@ -206,6 +219,10 @@ def run_served(cfa, n, rate, seed, budget):
             if not np.all(np.isfinite(u)):
                 raise SystemExit(f"non-finite served result {i}")
             lat[i] = now - arrivals[i]
+        if now > next_report:
+            print(f"bench: serving t={now:.0f}s submitted {submitted}/{n} done {len(lat)} running {running} "
+                  f"queued {queued}", file=sys.stderr, flush=True)
+            next_report += 5.0
         if running == 0 and queued == 0 and submitted < n:
             time.sleep(max(0.0, min(0.01, arrivals[submitted] - now)))
     wall = time.perf_counter() - t0
@@ -213,7 +230,27 @@ def run_served(cfa, n, rate, seed, budget):
     lats = np.array([lat[i] for i in range(n)])
     return dict(value=work / wall, unit="grid-pt*steps/s", requests=n, arrival_rate_per_s=rate,
                 p50_latency_s=float(np.percentile(lats, 50)), p99_latency_s=float(np.percentile(lats, 99)),
-                wall_s=wall, max_concurrent=max_running, hbm_budget_bytes=budget)
+                wall_s=wall, max_concurrent=max_running, hbm_budget_bytes=budget, work_grid_pt_steps=float(work))
+
+
+def plain_for_requests(cfa, n, seed, steps, warmup):
+    """The served request mix on the plain single-field path: requests one
+    after another, each at the per-step time of its (grid, flow) class alone."""
+    reqs = serve_requests(n, seed)
+    per = {}
+    for q in reqs:
+        key = (q["grid"], q["flags"]["nonlinearity"])
+        if key in per:
+            continue
+        dns = cfa.DNS(flow_fields(cfa, q["grid"], member_field(q)), cfa.DNSFlags(**q["flags"]))
+        dns.advance_resident(warmup)
+        t0 = time.perf_counter()
+        dns.advance_resident(steps)
+        per[key] = (time.perf_counter() - t0) / steps
+    t = sum(q["horizon"] * per[(q["grid"], q["flags"]["nonlinearity"])] for q in reqs)
+    work = sum(np.prod(q["grid"]) * q["horizon"] for q in reqs)
+    return dict(value=work / t, seconds=t,
+                ms_per_step_by_class={f"{'x'.join(map(str, k[0]))} {k[1]}": round(v * 1e3, 4) for k, v in per.items()})
 
 
 def run_single_c2(cfa, steps, warmup):
@@ -276,8 +313,9 @@ def cpu_baseline(members, steps_timed=2, budget_s=25.0):
 
 
 def roofline(members, kt, steps):
-    """Dominant kernel of the batched run: algorithmic bytes over its launches
-    in the timed region / their summed HIP-event durations."""
+    """Dominant kernel of the batched run: algorithmic bytes of its launches
+    over `steps` profiled steps / their summed HIP-event durations (events
+    recorded on each class's own stream around every launch)."""
     totals = {}
     for mem in members:
         for k, b in kernel_bytes(mem["grid"], mem["flags"]["nonlinearity"]).items():
@@ -304,8 +342,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=48)
-    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--serve-requests", type=int, default=96)
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--serve-budget", type=float, default=64e9, help="HBM budget of the server, bytes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run plain device copies of known size (for scaling HBM counters)")
@@ -331,7 +369,8 @@ def main():
              "integrations at 32x49x32, skew-symmetric NL, bulk velocity, SBDF3 dt=0.02")
     print(f"bench: {args.workload} batched, {len(members)} members, {args.steps} timed steps", file=sys.stderr,
           flush=True)
-    b = run_batched(cfa, members, args.steps, args.warmup)
+    prof_steps = max(10, min(50, args.steps))
+    b = run_batched(cfa, members, args.steps, args.warmup, prof_steps)
     print(f"bench: batched {b['value']:.4g} grid-pt*steps/s, {b['wall'] / args.steps * 1e3:.3f} ms/step",
           file=sys.stderr, flush=True)
     out = {
@@ -349,8 +388,10 @@ def main():
         "data": "synthetic (seeded divergence-free perturbations)",
         "config": {"workload": label, "members": len(members), "grid_classes": b["classes"],
                    "grid_points_per_step": int(b["pts"]), "parallelism": "batched members, one GPU"},
-        "roofline": roofline(members, b["kt"], args.steps),
-        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in b["kt"].items() if v[1] > 0},
+        "graphs": b["graphs"],
+        "roofline": roofline(members, b["kt"], prof_steps),
+        "kernels_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in b["kt"].items() if v[1] > 0},
+        "kernel_profile_steps": prof_steps,
     }
     B_step = sum(16.0 * 25 * live(m["grid"]) for m in members)
     out["step_roofline"] = {"bytes_per_step": B_step, "achieved": B_step / (b["wall"] / args.steps) / 1e9,
@@ -366,9 +407,12 @@ def main():
     if not args.no_serve:
         print("bench: serving", file=sys.stderr, flush=True)
         out["served"] = run_served(cfa, args.serve_requests, args.serve_rate, 4242, args.serve_budget)
-        if "plain_gpu" in out:
-            out["served"]["served_over_plain"] = out["served"]["value"] / out["plain_gpu"]["value"]
         print(f"bench: served {json.dumps(out['served'])}", file=sys.stderr, flush=True)
+        if not args.no_plain:
+            print("bench: served request mix on the plain path", file=sys.stderr, flush=True)
+            sp = plain_for_requests(cfa, args.serve_requests, 4242, 20, 5)
+            out["served"]["plain_gpu_same_requests"] = sp
+            out["served"]["served_over_plain"] = out["served"]["value"] / sp["value"]
     if not args.no_c2:
         print("bench: single 192x129x192", file=sys.stderr, flush=True)
         out["single_c2"] = run_single_c2(cfa, 50, 10)

@@ -31,7 +31,7 @@ EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_des
             "cfg_kernel_name", "cfg_nonlinear", "cfg_solve",
             "cfg_batch_create", "cfg_batch_destroy", "cfg_batch_add", "cfg_batch_remove", "cfg_batch_advance",
             "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_dpdx",
-            "cfg_batch_profile", "cfg_batch_kernel_times",
+            "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats",
             "cfg_server_create", "cfg_server_destroy", "cfg_server_submit", "cfg_server_step", "cfg_server_result",
             "cfg_server_memory")
 NKERNELS = 9
@@ -102,6 +102,8 @@ def library():
         "cfg_batch_dpdx": ([v, i, P], i),
         "cfg_batch_profile": ([v, i], i),
         "cfg_batch_kernel_times": ([v, P, C.POINTER(C.c_longlong)], i),
+        "cfg_batch_set_graphs": ([v, i], i),
+        "cfg_batch_graph_stats": ([v, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], i),
         "cfg_server_create": ([d, C.POINTER(v)], i),
         "cfg_server_destroy": ([v], i),
         "cfg_server_submit": ([v, G, F, P, P, i, C.POINTER(C.c_longlong)], i),
@@ -390,6 +392,16 @@ class Batch:
         cnt = (C.c_longlong * NKERNELS)()
         _check(library().cfg_batch_kernel_times(self._h, _ptr(tot), cnt))
         return {library().cfg_kernel_name(i).decode(): (float(tot[i]), int(cnt[i])) for i in range(NKERNELS)}
+
+    def set_graphs(self, enable=True):
+        """hipGraph capture and replay of repeating ticks (on by default)."""
+        _check(library().cfg_batch_set_graphs(self._h, int(bool(enable))))
+
+    def graph_stats(self):
+        """(captures, replays) summed over the grid classes."""
+        c, r = C.c_longlong(), C.c_longlong()
+        _check(library().cfg_batch_graph_stats(self._h, C.byref(c), C.byref(r)))
+        return c.value, r.value
 
 
 class Server:

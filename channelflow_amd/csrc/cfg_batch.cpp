@@ -45,6 +45,7 @@ class Batch {
         if (cit == classes_.end()) {
             auto c = std::make_unique<BatchClass>();
             c->ex = std::make_unique<Exec>(grid, flags.dealiasing == CFG_DealiasXZ);
+            c->ex->set_graphs(graphs_);
             cit = classes_.emplace(k, std::move(c)).first;
         }
         BatchClass* cls = cit->second.get();
@@ -127,7 +128,23 @@ class Batch {
     }
     int nclasses() const { return (int)classes_.size(); }
 
+    // hipGraph capture and replay of each class's repeating ticks (on by default)
+    void set_graphs(bool on) {
+        graphs_ = on;
+        for (auto& kv : classes_) kv.second->ex->set_graphs(on);
+    }
+    void graph_stats(long long* captures, long long* replays) const {
+        long long c = 0, r = 0;
+        for (auto& kv : classes_) {
+            c += kv.second->ex->graph_captures();
+            r += kv.second->ex->graph_replays();
+        }
+        if (captures) *captures = c;
+        if (replays) *replays = r;
+    }
+
    private:
+    bool graphs_ = true;
     std::map<GridKey, std::unique_ptr<BatchClass>> classes_;
     std::map<int, std::unique_ptr<BatchMember>> members_;
     std::deque<TickOps> ops_;
@@ -385,6 +402,20 @@ int cfg_batch_kernel_times(cfg_batch* b, double* total_ms, long long* counts) {
     return guarded([&] {
         if (!b) throw Error(CFG_ERR_ARG, "null batch");
         b->b.kernel_times(total_ms, counts);
+    });
+}
+
+int cfg_batch_set_graphs(cfg_batch* b, int enable) {
+    return guarded([&] {
+        if (!b) throw Error(CFG_ERR_ARG, "null batch");
+        b->b.set_graphs(enable != 0);
+    });
+}
+
+int cfg_batch_graph_stats(const cfg_batch* b, long long* captures, long long* replays) {
+    return guarded([&] {
+        if (!b) throw Error(CFG_ERR_ARG, "null batch");
+        b->b.graph_stats(captures, replays);
     });
 }
 

@@ -464,6 +464,7 @@ class TableCache {
             HIPCHK(hipStreamSynchronize(st));
             map_.clear();
             used_ = 0;
+            ++gen_;
         }
         char* dst = reinterpret_cast<char*>(arena_.p) + used_;
         used_ += need;
@@ -476,10 +477,14 @@ class TableCache {
     const T* get(const std::vector<T>& v, hipStream_t st) {
         return static_cast<const T*>(get(v.data(), v.size() * sizeof(T), st));
     }
+    // changes whenever the arena is recycled, i.e. when a table address may
+    // start to hold other bytes
+    unsigned long generation() const { return gen_; }
 
    private:
     DBuf arena_;
     size_t cap_, used_ = 0;
+    unsigned long gen_ = 0;
     std::unordered_map<std::string, void*> map_;
 };
 

@@ -17,6 +17,8 @@
 
 #include <algorithm>
 
+#include <map>
+
 #include "cfg_kernels.hpp"
 #include "cfg_math.hpp"
 #include "cfg_wave.hpp"
@@ -1472,9 +1474,16 @@ static int xfft_lines(const Geom& g, const NLTune& tn) {
     return tn.xl > 0 ? std::min(tn.xl, m) : std::min(4, m);
 }
 
+// Raises a kernel's dynamic-LDS limit once per (kernel, size), so that launches
+// recorded into a graph make no attribute calls.
 template <class K>
 static void allow_lds(K kernel, size_t bytes) {
-    if (bytes > 64 * 1024) (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (bytes <= 64 * 1024) return;
+    static std::map<const void*, size_t> done;
+    size_t& d = done[(const void*)kernel];
+    if (d >= bytes) return;
+    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    d = bytes;
 }
 
 #define CFG_FFT_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)

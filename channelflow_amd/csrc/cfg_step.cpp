@@ -1,5 +1,7 @@
 // Tick form of channelflow's time stepping (dns.cpp, dnsalgo.cpp, nse.cpp)
 // and the batched tick executor.
+#include <cstring>
+
 #include "cfg_step.hpp"
 
 namespace cfg {
@@ -409,6 +411,7 @@ Exec::Exec(const cfg_grid& grid, bool dealias) {
 Exec::~Exec() {
     if (st) {
         (void)hipStreamSynchronize(st);
+        drop_graphs();
         kt.reset();
         (void)hipStreamDestroy(st);
     }
@@ -425,16 +428,44 @@ void Exec::reserve(int n) {
     zo_.alloc((size_t)c * nlwork_x_doubles(g));
     lscr_.alloc((size_t)c * 3 * g.Ny * g.M);
     cap_ = c;
+    ++buf_gen_;
 }
 
-void Exec::run(const std::vector<TickOps*>& ops) {
-    if (ops.empty()) return;
+// sequences (seen or captured) kept per class
+static constexpr size_t kMaxGraphKeys = 256;
+
+void Exec::set_graphs(bool on) {
+    sync();
+    graphs_on_ = on;
+    if (!on) drop_graphs();
+}
+
+void Exec::drop_graphs() {
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
+    seen_.clear();
+}
+
+// Resolves a tick into launch records: member tables are made resident (a
+// copy only for bytes not seen before) and a nonlinear form's tiling is tuned
+// on its first launch at a member-count bucket. Nothing here is captured.
+void Exec::plan(const std::vector<TickOps*>& ops) {
     const Geom& g = geo->g;
-    reserve((int)ops.size());
+    recs_.clear();
+    auto rec = [&](LaunchRec::Kind k) -> LaunchRec& {
+        recs_.emplace_back();
+        LaunchRec& r = recs_.back();
+        std::memset(&r, 0, sizeof r);  // padding too: the record bytes are the graph key
+        r.kind = k;
+        return r;
+    };
     for (TickOps* o : ops)
-        for (int i = 0; i < o->ncopy; ++i)
-            HIPCHK(hipMemcpyAsync(o->copy[i].dst, o->copy[i].src, o->copy[i].n * sizeof(double),
-                                  hipMemcpyDeviceToDevice, st));
+        for (int i = 0; i < o->ncopy; ++i) {
+            LaunchRec& r = rec(LaunchRec::COPY);
+            r.dst = o->copy[i].dst;
+            r.src = o->copy[i].src;
+            r.bytes = o->copy[i].n * sizeof(double);
+        }
     const YMats ym = geo->ymats();
     const NLWork w{yp_.p, xp_.p, zo_.p, fo_.p};
     for (int m : {NL_ROT, NL_CONV, NL_DIV, NL_SKEW}) {
@@ -454,13 +485,22 @@ void Exec::run(const std::vector<TickOps*>& ops) {
         if (tk == tune_.end())  // first launch of this form at this batch size
             tk = tune_.emplace(std::make_pair(m, nb),
                                tune_nonlinear(g, ym, geo->fx.plan, geo->fz.plan, m, n, tab, rotating, w, st)).first;
-        launch_nonlinear(g, ym, geo->fx.plan, geo->fz.plan, m, n, tab, rotating, w, tk->second, st, &kt);
+        LaunchRec& r = rec(LaunchRec::NONLINEAR);
+        r.method = m;
+        r.n = n;
+        r.tab = tab;
+        r.rotating = rotating;
+        r.tn = tk->second;
     }
     for (TickOps* o : ops)
         if (o->comb) {
-            hipEvent_t ev = kt.begin();
-            launch_combine(nfield, o->cn, o->ccoef, o->cin, o->cout, st);
-            kt.end(K_COMBINE, ev);
+            LaunchRec& r = rec(LaunchRec::COMBINE);
+            r.cn = o->cn;
+            for (int j = 0; j < o->cn; ++j) {
+                r.ccoef[j] = o->ccoef[j];
+                r.cin[j] = o->cin[j];
+            }
+            r.cout = o->cout;
         }
     lt_.clear();
     for (TickOps* o : ops)
@@ -470,19 +510,104 @@ void Exec::run(const std::vector<TickOps*>& ops) {
             lt_.push_back(l);
         }
     if (!lt_.empty()) {
-        hipEvent_t ev = kt.begin();
-        launch_linear(g, (int)lt_.size(), cache.get(lt_, st), st);
-        kt.end(K_LINEAR, ev);
+        LaunchRec& r = rec(LaunchRec::LINEAR);
+        r.n = (int)lt_.size();
+        r.tab = cache.get(lt_, st);
     }
     tt_.clear();
     for (TickOps* o : ops)
         if (o->solve) tt_.push_back(o->tau);
     if (!tt_.empty()) {
-        hipEvent_t ev = kt.begin();
-        launch_tau_solve(g, (int)tt_.size(), cache.get(tt_, st), st);
-        kt.end(K_TAU, ev);
+        LaunchRec& r = rec(LaunchRec::TAU);
+        r.n = (int)tt_.size();
+        r.tab = cache.get(tt_, st);
+    }
+}
+
+void Exec::issue() {
+    const Geom& g = geo->g;
+    const YMats ym = geo->ymats();
+    const NLWork w{yp_.p, xp_.p, zo_.p, fo_.p};
+    for (const LaunchRec& r : recs_) {
+        switch (r.kind) {
+            case LaunchRec::COPY:
+                HIPCHK(hipMemcpyAsync(r.dst, r.src, r.bytes, hipMemcpyDeviceToDevice, st));
+                break;
+            case LaunchRec::NONLINEAR:
+                launch_nonlinear(g, ym, geo->fx.plan, geo->fz.plan, r.method, r.n,
+                                 static_cast<const NLMember*>(r.tab), r.rotating != 0, w, r.tn, st, &kt);
+                break;
+            case LaunchRec::COMBINE: {
+                hipEvent_t ev = kt.begin();
+                launch_combine(nfield, r.cn, r.ccoef, r.cin, r.cout, st);
+                kt.end(K_COMBINE, ev);
+                break;
+            }
+            case LaunchRec::LINEAR: {
+                hipEvent_t ev = kt.begin();
+                launch_linear(g, r.n, static_cast<const LinearMember*>(r.tab), st);
+                kt.end(K_LINEAR, ev);
+                break;
+            }
+            case LaunchRec::TAU: {
+                hipEvent_t ev = kt.begin();
+                launch_tau_solve(g, r.n, static_cast<const TauMember*>(r.tab), st);
+                kt.end(K_TAU, ev);
+                break;
+            }
+        }
     }
     HIPCHK(hipGetLastError());
+}
+
+void Exec::run(const std::vector<TickOps*>& ops) {
+    if (ops.empty()) return;
+    reserve((int)ops.size());
+    plan(ops);
+    if (!graphs_on_ || kt.on) {  // timing needs the events between launches
+        issue();
+        return;
+    }
+    const unsigned long gen = buf_gen_ * 1000003UL + cache.generation();
+    if (gen != graph_gen_) {
+        HIPCHK(hipStreamSynchronize(st));  // replays in flight may hold the old graphs
+        drop_graphs();
+        graph_gen_ = gen;
+    }
+    std::string key(reinterpret_cast<const char*>(recs_.data()), recs_.size() * sizeof(LaunchRec));
+    auto it = graphs_.find(key);
+    if (it != graphs_.end()) {
+        HIPCHK(hipGraphLaunch(it->second, st));
+        ++replays_;
+        return;
+    }
+    if (!seen_.count(key)) {  // first occurrence: eager
+        if (seen_.size() + graphs_.size() >= kMaxGraphKeys) {  // membership churn: start over
+            HIPCHK(hipStreamSynchronize(st));
+            drop_graphs();
+        }
+        seen_.insert(std::move(key));
+        issue();
+        return;
+    }
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        issue();
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(st, &gr));
+    hipGraphExec_t ge = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    HIPCHK(e);
+    graphs_.emplace(key, ge);
+    seen_.erase(key);
+    ++captures_;
+    HIPCHK(hipGraphLaunch(ge, st));
 }
 
 }  // namespace cfg

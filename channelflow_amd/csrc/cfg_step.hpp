@@ -11,6 +11,9 @@
 // is an Exec per grid class with many.
 #pragma once
 
+#include <unordered_map>
+#include <unordered_set>
+
 #include "cfg_host.hpp"
 
 namespace cfg {
@@ -142,7 +145,29 @@ class Integrator {
     long steps_ = 0;
 };
 
+// One launch of a tick, with its tables already resident on the device.
+struct LaunchRec {
+    enum Kind { COPY, NONLINEAR, COMBINE, LINEAR, TAU } kind;
+    int method, n, rotating, cn;
+    NLTune tn;
+    const void* tab;
+    void* dst;
+    const void* src;
+    size_t bytes;
+    double ccoef[MAX_TERMS];
+    const double* cin[MAX_TERMS];
+    double* cout;
+};
+
 // Runs ticks of integrations on one grid, batched, on one stream.
+//
+// With graphs on, a tick whose launch sequence (kernels, arguments and table
+// addresses) has been seen before is captured into a hipGraph on its second
+// occurrence and replayed from then on. Multistep histories rotate with a
+// fixed period, so a class of members in steady state cycles through a few
+// sequences. Graphs are dropped when the work buffers are reallocated or the
+// launch-table arena is recycled, the two events that move an address a graph
+// holds.
 class Exec {
    public:
     explicit Exec(const cfg_grid& grid, bool dealias);
@@ -151,6 +176,9 @@ class Exec {
     Exec& operator=(const Exec&) = delete;
     void run(const std::vector<TickOps*>& ops);
     void sync() { HIPCHK(hipStreamSynchronize(st)); }
+    void set_graphs(bool on);
+    long graph_replays() const { return replays_; }
+    long graph_captures() const { return captures_; }
 
     std::shared_ptr<Geometry> geo;
     hipStream_t st = nullptr;
@@ -162,12 +190,21 @@ class Exec {
 
    private:
     void reserve(int n);
+    void plan(const std::vector<TickOps*>& ops);  // fills recs_
+    void issue();                                  // launches recs_ on st
+    void drop_graphs();
     int cap_ = 0;
     DBuf yp_, xp_, zo_, fo_, lscr_;
     std::vector<NLMember> nlt_;
     std::vector<LinearMember> lt_;
     std::vector<TauMember> tt_;
+    std::vector<LaunchRec> recs_;
     std::map<std::pair<int, int>, NLTune> tune_;  // (nonlinear form, member-count bucket) -> tiling
+    bool graphs_on_ = false;
+    unsigned long buf_gen_ = 0, graph_gen_ = ~0UL;  // buffer generation; generation the graphs belong to
+    std::unordered_map<std::string, hipGraphExec_t> graphs_;
+    std::unordered_set<std::string> seen_;
+    long replays_ = 0, captures_ = 0;
 };
 
 }  // namespace cfg

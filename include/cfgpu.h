@@ -137,6 +137,12 @@ int cfg_batch_dpdx(cfg_batch* b, int member, double* dPdx);
 /* Kernel timing summed over the batch's streams (as cfg_dns_profile). */
 int cfg_batch_profile(cfg_batch* b, int enable);
 int cfg_batch_kernel_times(cfg_batch* b, double* total_ms, long long* counts);
+/* hipGraph capture of each grid class's repeating tick (no counterpart in the
+ * reference; on by default). A tick whose launch sequence was seen before is
+ * captured on its second occurrence and replayed afterwards; results are
+ * bit-identical to eager launches. Profiling switches replay off while it is on. */
+int cfg_batch_set_graphs(cfg_batch* b, int enable);
+int cfg_batch_graph_stats(const cfg_batch* b, long long* captures, long long* replays);
 
 /* ---- Serving ----------------------------------------------------------------
  * Continuous batching of integration requests (no counterpart in the
