@@ -342,6 +342,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--c2-only", action="store_true",
+                    help="run only the plain 192x129x192 channel and print its line (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=96)
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--serve-budget", type=float, default=64e9, help="HBM budget of the server, bytes")
@@ -362,6 +364,9 @@ def main():
         sys.exit(2)
 
     import channelflow_amd as cfa
+    if args.c2_only:
+        print(json.dumps({"single_c2": run_single_c2(cfa, args.steps, args.warmup)}), flush=True)
+        return
     members = c3_members() if args.workload == "c3" else c4_members()
     label = ("C3 ensemble: 64 plane Couette Re=400 integrations (24x 32x33x32, 24x 48x49x48, 16x 64x65x64), "
              "skew-symmetric NL, bulk velocity, SBDF3 dt=0.02" if args.workload == "c3" else
@@ -467,7 +472,7 @@ def _counter_rows(path, counter):
             if r["Counter_Name"] != counter:
                 continue
             name = r["Kernel_Name"]
-            m = re.search(r"cfg::(k_\w+?)(?:_wave)?(?:<|\()", name)
+            m = re.search(r"cfg::(k_\w+?)(?:_wave|_w|_t)?(?:<|\()", name)  # variants count as their kernel
             rows.append((m.group(1) if m else name.split("(")[0].strip(), float(r["Counter_Value"])))
     return rows
 
