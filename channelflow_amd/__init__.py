@@ -31,7 +31,7 @@ EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_des
             "cfg_kernel_name", "cfg_nonlinear", "cfg_solve",
             "cfg_batch_create", "cfg_batch_destroy", "cfg_batch_add", "cfg_batch_remove", "cfg_batch_advance",
             "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_dpdx",
-            "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats",
+            "cfg_batch_cfl", "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats",
             "cfg_server_create", "cfg_server_destroy", "cfg_server_submit", "cfg_server_step", "cfg_server_result",
             "cfg_server_memory")
 NKERNELS = 9
@@ -100,6 +100,7 @@ def library():
         "cfg_batch_set_state": ([v, i, P, P], i),
         "cfg_batch_time": ([v, i, P], i),
         "cfg_batch_dpdx": ([v, i, P], i),
+        "cfg_batch_cfl": ([v, i, P], i),
         "cfg_batch_profile": ([v, i], i),
         "cfg_batch_kernel_times": ([v, P, C.POINTER(C.c_longlong)], i),
         "cfg_batch_set_graphs": ([v, i], i),
@@ -160,6 +161,14 @@ class DNSFlags:
         self.wupperwall = 0.0
         self.t0 = 0.0
         self.dt = 0.03125
+        # time-step control, used on the host by TimeStep (dnsflags.h:86-87, 129-136)
+        self.T = 20.0
+        self.dT = 1.0
+        self.variabledt = True
+        self.dtmin = 0.001
+        self.dtmax = 0.2
+        self.CFLmin = 0.4
+        self.CFLmax = 0.6
         for k, val in kw.items():
             if not hasattr(self, k):
                 raise CfgError(f"unknown DNSFlags field {k}")
@@ -289,7 +298,16 @@ class DNS:
 
     def reset_dt(self, dt):
         _check(library().cfg_dns_reset_dt(self._h, float(dt)))
-        self.flags = self.flags.copy(dt=dt)
+        self.flags = self.flags.copy(dt=float(dt))
+
+    def CFL(self, u=None):
+        """DNS::CFL(u) (dns.cpp:282-289, dnsalgo.cpp:63-67) of the velocity this
+        DNS holds. u, if given, must be that velocity (fields[0] after advance,
+        as every reference caller passes, simulateflow.cpp:140-142); it is
+        accepted for the reference's signature, not read."""
+        if u is not None and tuple(u.data.shape) != tuple(self._shape_u):
+            raise CfgError("CFL(u): u is not this DNS's velocity field")
+        return self._scalar(library().cfg_dns_cfl)
 
     def reset_time(self, t):
         _check(library().cfg_dns_reset_time(self._h, float(t)))
@@ -382,6 +400,12 @@ class Batch:
     def dPdx(self, member):
         v = C.c_double()
         _check(library().cfg_batch_dpdx(self._h, int(member), C.byref(v)))
+        return v.value
+
+    def CFL(self, member):
+        """DNS::CFL of the member's current velocity."""
+        v = C.c_double()
+        _check(library().cfg_batch_cfl(self._h, int(member), C.byref(v)))
         return v.value
 
     def profile(self, enable=True):
@@ -485,3 +509,6 @@ def device_count():
     n = C.c_int()
     _check(library().cfg_device_count(C.byref(n)))
     return n.value
+
+
+from .timestep import TimeStep  # noqa: E402  (host-side dt control)

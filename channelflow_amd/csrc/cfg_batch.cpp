@@ -405,6 +405,14 @@ int cfg_batch_kernel_times(cfg_batch* b, double* total_ms, long long* counts) {
     });
 }
 
+int cfg_batch_cfl(cfg_batch* b, int member, double* cfl) {
+    return guarded([&] {
+        if (!b || !cfl) throw Error(CFG_ERR_ARG, "null argument");
+        BatchMember& m = b->b.get(member);
+        *cfl = m.cls->ex->cfl({m.it->cfl_member()})[0] * m.it->cfl_scale();
+    });
+}
+
 int cfg_batch_set_graphs(cfg_batch* b, int enable) {
     return guarded([&] {
         if (!b) throw Error(CFG_ERR_ARG, "null batch");

@@ -1114,6 +1114,71 @@ __global__ void __launch_bounds__(256) k_yfwd_t(Geom g, YMats ym, int have_g, co
     }
 }
 
+// ---- CFL factor -------------------------------------------------------------
+// FlowField::CFLfactor(Ubase, Wbase) as DNSAlgorithm::CFL calls it
+// (flowfield.cpp:4035-4068, dnsalgo.cpp:63-67): the largest of
+// (u_i + U_i(y)) / dx_i over every grid point and component, signed as in the
+// reference, floored at 0; dx_y is the local Chebyshev spacing. The physical
+// velocity comes from the divergence form's inverse transforms, which add U, W
+// and -Vsuck to the (0,0) mode; Vsuck is added back for v. One partial maximum
+// per block, then one per member.
+template <int NZ>
+__global__ void __launch_bounds__(256) k_zcfl(Geom g, FFTPlan pz, int XB, const NLMember* __restrict__ tab,
+                                              const double2* __restrict__ xp0, long sx, double* __restrict__ part) {
+    extern __shared__ double2 lds[];
+    __shared__ double red[4];
+    const double vs = tab[blockIdx.y].Vsuck;
+    const double2* __restrict__ xp = xp0 + blockIdx.y * sx;
+    const int nxc = (g.Nx + XB - 1) / XB;
+    ZCtx c;
+    c.g = g;
+    c.pz = pz;
+    c.y = blockIdx.x / nxc;
+    c.x0 = (blockIdx.x % nxc) * XB;
+    c.nl = min(XB, g.Nx - c.x0);
+    c.XB = XB;
+    c.tw = lds;
+    c.wa = lds + g.Nz;
+    c.wb = c.wa + XB * lpitch(g.Nz);
+    c.phys = reinterpret_cast<double*>(c.wb + XB * lpitch(g.Nz));
+    stage_tw(c.tw, pz);
+    zc2r_pair<NZ>(c, xp, 0, false, 0, 1, false, 1);
+    zc2r_pair<NZ>(c, xp, 2, false, 2, -1, false, -1);
+    const int Ny = g.Ny, y = c.y;
+    const double piN = PI / (Ny - 1), rad = (g.b - g.a) / 2, cen = (g.b + g.a) / 2;  // chebypoints (chebyshev.cpp:746-754)
+    auto yc = [&](int j) { return cen + rad * cos(j * piN); };
+    const double dy = (y == 0 || y == Ny - 1) ? yc(0) - yc(1) : (yc(y - 1) - yc(y + 1)) / 2.0;
+    const double dx = g.Lx / g.Nx, dz = g.Lz / g.Nz;
+    const double *u0 = physf(c, 0), *u1 = physf(c, 1), *u2 = physf(c, 2);
+    double m = 0.0;
+    for (int w = threadIdx.x; w < c.nl * g.Nz; w += blockDim.x) {
+        m = fmax(m, u0[w] / dx);
+        m = fmax(m, (u1[w] + vs) / dy);
+        m = fmax(m, u2[w] / dz);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, red[i]);
+        part[(long)blockIdx.y * gridDim.x + blockIdx.x] = m;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cfl_reduce(int nb, const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double red[4];
+    const double* p = part + (long)blockIdx.x * nb;
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) m = fmax(m, p[i]);
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmax(m, red[i]);
+        out[blockIdx.x] = m;
+    }
+}
+
 // ---- y transforms on the FP64 matrix cores ----------------------------------
 // v_mfma_f64_16x16x4f64 (one wave: D = A B + C, A 16x4, B 4x16, D 16x16).
 // Operand layout, lane l: A[l % 16][l / 16], B[l / 16][l % 16]; result
@@ -1548,6 +1613,36 @@ static void launch_z_any(int method, const Geom& g, const FFTPlan& pz, int n, co
         case NL_SKEW: launch_z<NL_SKEW>(g, pz, n, tab, want_g, xp, zo, sx, tn, st); break;
         default: launch_z<NL_DIV>(g, pz, n, tab, want_g, xp, zo, sx, tn, st);
     }
+}
+
+template <int NZ>
+static void launch_zcfl_t(const Geom& g, const FFTPlan& pz, int n, const NLMember* tab, const double2* xp, long sx,
+                          double* part, int XB, hipStream_t st) {
+    const size_t lds = (size_t)g.Nz * sizeof(double2) +
+                       (size_t)XB * (2 * lpitch(g.Nz) * sizeof(double2) + (size_t)g.Nz * 3 * sizeof(double));
+    const dim3 grid(g.Ny * ((g.Nx + XB - 1) / XB), n);
+    allow_lds(k_zcfl<NZ>, lds);
+    k_zcfl<NZ><<<grid, 256, lds, st>>>(g, pz, XB, tab, xp, sx, part);
+}
+
+void launch_cfl(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int n, const NLMember* tab,
+                const NLWork& w, double* out, hipStream_t st) {
+    const long sy = nlwork_y_doubles(g) / 2, sx = nlwork_x_doubles(g) / 2;
+    double2* yp = reinterpret_cast<double2*>(w.yp);
+    double2* xp = reinterpret_cast<double2*>(w.xp);
+    double* part = w.zo;  // n x blocks partial maxima
+    launch_yinv<NL_DIV>(g, ym, n, tab, yp, sy, st);
+    launch_xfft(true, g, px, 3, 3, n, yp, sy, xp, sx, NLTune{}, st);
+    const int XB = zpass_xb_max(g, 3);
+    const int nb = g.Ny * ((g.Nx + XB - 1) / XB);
+    switch (g.Nz) {
+#define CASE_Z(N) \
+    case N: launch_zcfl_t<N>(g, pz, n, tab, xp, sx, part, XB, st); break;
+        CFG_FFT_SIZES(CASE_Z)
+#undef CASE_Z
+        default: launch_zcfl_t<0>(g, pz, n, tab, xp, sx, part, XB, st);
+    }
+    k_cfl_reduce<<<n, 256, 0, st>>>(nb, part, out);
 }
 
 // Nonlinear term of n members of one grid class. rotating: some member has a

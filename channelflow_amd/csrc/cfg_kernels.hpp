@@ -207,6 +207,12 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
 
 // Times the candidate tilings on these work buffers (contents overwritten)
 // and returns the fastest. Results are identical for every tiling.
+// Largest (u_i + U_i)/dx_i over the grid points of each of n fields, into
+// out[n] (device): FlowField::CFLfactor(Ubase, Wbase), flowfield.cpp:4035-4068.
+// Uses the work buffers of the nonlinear term.
+void launch_cfl(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int n, const NLMember* tab,
+                const NLWork& w, double* out, hipStream_t st);
+
 NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, int n,
                       const NLMember* tab, bool rotating, const NLWork& w, hipStream_t st);
 

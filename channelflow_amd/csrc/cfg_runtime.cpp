@@ -127,10 +127,10 @@ int cfg_dns_ubulk(const cfg_dns* dns, double* v) {
     });
 }
 
-int cfg_dns_cfl(cfg_dns* dns, double* cfl) {
+int cfg_dns_cfl(cfg_dns* dns, double* cfl) {  // DNS::CFL, dns.cpp:282-289
     return guarded([&] {
         if (!dns || !cfl) throw Error(CFG_ERR_ARG, "null argument");
-        throw Error(CFG_ERR_STATE, "CFL is not implemented on the device path yet");
+        *cfl = dns->it->exec().cfl({dns->it->cfl_member()})[0] * dns->it->cfl_scale();
     });
 }
 

@@ -387,6 +387,15 @@ void Integrator::reset_dt(double dt) {  // dns.cpp:180-185
     main_sub_ = 0;
 }
 
+NLMember Integrator::cfl_member() const {
+    return NLMember{cur_.u, nullptr, model_->base_values(), flags_.Vsuck, flags_.rotation};
+}
+
+double Integrator::cfl_scale() const {
+    const double pi = 3.14159265358979323846;
+    return flags_.dealiasing == CFG_DealiasXZ ? 2.0 * pi / 3.0 * flags_.dt : pi * flags_.dt;
+}
+
 void Integrator::reset_time(double t) {
     main_->reset_time(t);
     if (init_) init_->reset_time(t);
@@ -444,6 +453,21 @@ void Exec::drop_graphs() {
     for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
     graphs_.clear();
     seen_.clear();
+}
+
+std::vector<double> Exec::cfl(const std::vector<NLMember>& fields) {
+    const int n = (int)fields.size();
+    std::vector<double> out(n, 0.0);
+    if (n == 0) return out;
+    reserve(n);
+    if (cflout_.n < (size_t)n) cflout_.alloc(n);
+    const NLMember* tab = cache.get(fields, st);
+    const NLWork w{yp_.p, xp_.p, zo_.p, fo_.p};
+    launch_cfl(geo->g, geo->ymats(), geo->fx.plan, geo->fz.plan, n, tab, w, cflout_.p, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out.data(), cflout_.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return out;
 }
 
 // Resolves a tick into launch records: member tables are made resident (a

@@ -125,6 +125,8 @@ class Integrator {
     void reset_time(double t);
     void reset_gradp(double dPdx, double dPdz);
     void reset_bulkv(double Ubulk, double Wbulk);
+    NLMember cfl_member() const;  // the current velocity, for Exec::cfl
+    double cfl_scale() const;     // DNSAlgorithm::CFL's factor on CFLfactor (dnsalgo.cpp:63-67)
     double dPdx() { return main_nse_->dPdx(); }
     double Ubulk() const { return model_->UbulkAct(); }
     const FlowModel& model() const { return *model_; }
@@ -175,6 +177,8 @@ class Exec {
     Exec(const Exec&) = delete;
     Exec& operator=(const Exec&) = delete;
     void run(const std::vector<TickOps*>& ops);
+    // CFL factors (flowfield.cpp:4035-4068) of the given fields, synchronously
+    std::vector<double> cfl(const std::vector<NLMember>& fields);
     void sync() { HIPCHK(hipStreamSynchronize(st)); }
     void set_graphs(bool on);
     long graph_replays() const { return replays_; }
@@ -194,7 +198,7 @@ class Exec {
     void issue();                                  // launches recs_ on st
     void drop_graphs();
     int cap_ = 0;
-    DBuf yp_, xp_, zo_, fo_, lscr_;
+    DBuf yp_, xp_, zo_, fo_, lscr_, cflout_;
     std::vector<NLMember> nlt_;
     std::vector<LinearMember> lt_;
     std::vector<TauMember> tt_;

@@ -134,6 +134,8 @@ int cfg_batch_get_state(cfg_batch* b, int member, double* u_spec, double* q_spec
 int cfg_batch_set_state(cfg_batch* b, int member, const double* u_spec, const double* q_spec);
 int cfg_batch_time(cfg_batch* b, int member, double* t);
 int cfg_batch_dpdx(cfg_batch* b, int member, double* dPdx);
+/* DNS::CFL(u) of one member's current velocity (dns.cpp:282-289). */
+int cfg_batch_cfl(cfg_batch* b, int member, double* cfl);
 /* Kernel timing summed over the batch's streams (as cfg_dns_profile). */
 int cfg_batch_profile(cfg_batch* b, int enable);
 int cfg_batch_kernel_times(cfg_batch* b, double* total_ms, long long* counts);

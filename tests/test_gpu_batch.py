@@ -81,6 +81,22 @@ def test_graph_replay_matches_eager(gpu):
     assert st_g[0] > 0 and st_g[1] > 0
 
 
+def test_batch_cfl_matches_single(gpu):
+    """DNS::CFL of batch members equals their own single DNS's, bit for bit."""
+    b = cfa.Batch()
+    ids, dnss = [], []
+    for i in (0, 1, 4):
+        grid, fl, u0 = case(i)
+        ids.append(b.add(fields(grid, u0), cfa.DNSFlags(**fl)))
+        d = cfa.DNS(fields(grid, u0), cfa.DNSFlags(**fl))
+        d.advance_resident(7)
+        dnss.append(d)
+    b.advance(7)
+    for mid, d in zip(ids, dnss):
+        print(f"member {mid}: batch CFL {b.CFL(mid):.15e} single {d.CFL():.15e}", flush=True)
+        assert b.CFL(mid) == d.CFL()
+
+
 def test_batch_matches_single_and_oracle(gpu):
     n = 20
     b = cfa.Batch()

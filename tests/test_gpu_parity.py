@@ -124,6 +124,65 @@ def test_dns_schemes(gpu, method, stepping, init):
     assert np.all(eu <= 1e-10), eu
 
 
+CFL_CASES = {
+    "couette_skew": (C1_FLAGS, (24, 25, 24)),
+    "poiseuille_rot_nodealias": (dict(baseflow="LaminarBase", constraint="PressureGradient", dPdx=-3.0 / 2000,
+                                      nu=1 / 2000, dt=0.01, timestepping="SBDF3", initstepping="SMRK2",
+                                      nonlinearity="Rotational", dealiasing="NoDealiasing", taucorrection=True),
+                                 (16, 17, 16)),
+    "suction_rot": (dict(baseflow="SuctionBase", constraint="PressureGradient", dPdx=0.0, nu=0.0025, Vsuck=0.0025,
+                         dt=0.025, timestepping="SBDF3", initstepping="SMRK2", nonlinearity="Rotational",
+                         dealiasing="DealiasXZ", taucorrection=True), (24, 19, 24)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CFL_CASES))
+def test_cfl_matches_oracle(gpu, case):
+    """DNS::CFL (dnsalgo.cpp:63-67): the device's value on the same initial state <= 1e-12
+    relative to the oracle's, and again after 6 steps of each."""
+    flags, (Nx, Ny, Nz) = CFL_CASES[case]
+    u0 = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.2, decay=0.4, seed=31, dealias=flags["dealiasing"] != "NoDealiasing")
+    g, ur, qr, ug, qg, ref, dns = run_pair(Nx, Ny, Nz, LX_C, LZ_C, flags, u0, 0)
+    c0_ref, c0_gpu = ref.cfl(), dns.CFL()
+    ref.advance(6)
+    dns.advance_resident(6)
+    c6_ref, c6_gpu = ref.cfl(), dns.CFL()
+    print(f"{case}: CFL t=0 oracle {c0_ref:.15e} gpu {c0_gpu:.15e}; t=6dt oracle {c6_ref:.15e} gpu {c6_gpu:.15e}",
+          flush=True)
+    assert abs(c0_gpu - c0_ref) <= 1e-12 * abs(c0_ref)
+    assert abs(c6_gpu - c6_ref) <= 1e-12 * abs(c6_ref)
+
+
+def test_variable_dt_loop(gpu):
+    """simulateflow's loop (programs/simulateflow.cpp:131-142): advance n steps, CFL,
+    TimeStep.adjust, reset_dt on a change -- same dt sequence and states within
+    1e-10 of the oracle driven the same way."""
+    Nx, Ny, Nz = 24, 25, 24
+    flags = dict(C1_FLAGS, dt=0.05)
+    u0 = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.1, decay=0.3, seed=41)
+    g, _, _, _, _, ref, dns = run_pair(Nx, Ny, Nz, LX_C, LZ_C, flags, u0, 0)
+    ts_g = cfa.TimeStep(0.05, 0.001, 0.2, 0.5, 0.1, 0.2)
+    ts_r = cfa.TimeStep(0.05, 0.001, 0.2, 0.5, 0.1, 0.2)
+    changes = 0
+    for k in range(4):
+        dns.advance_resident(ts_g.n())
+        ref.advance(ts_r.n())
+        cg, cr = dns.CFL(), ref.cfl()
+        assert abs(cg - cr) <= 1e-10 * abs(cr)
+        ag, ar = ts_g.adjust(cg), ts_r.adjust(cr)
+        assert ag == ar and ts_g.n() == ts_r.n()
+        if ag:
+            dns.reset_dt(ts_g.dt())
+            ref.reset_dt(ts_r.dt())
+            changes += 1
+        print(f"interval {k}: CFL {cg:.6f} n {ts_g.n()} dt {ts_g.dt():.6f} t {dns.time():.6f}", flush=True)
+    assert changes >= 1
+    eu = rel_l2_per_component(g, dns.state()[0], ref.state()[0])
+    print("variable dt: rel L2 per component", eu, flush=True)
+    assert np.all(eu <= 1e-10), eu
+    assert abs(dns.time() - ref.time) < 1e-12
+
+
 def test_golden_trajectory(gpu):
     """timeIntegrationTest: uinit -> 440 SBDF3 steps -> ufinal, L2Dist <= 1e-13 (the reference's own bar)."""
     man = json.load(open(os.path.join(HERE, "golden", "manifest.json")))

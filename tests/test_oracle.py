@@ -17,6 +17,9 @@ Pins, by reference test:
     parabola is a steady state.
   * skew = (convection + divergence)/2 and rotational = convection - grad(|u|^2/2)
     identities for the nonlinear term.
+  * DNS::CFL (dnsalgo.cpp:63-67, flowfield.cpp:4035-4068): the laminar Couette
+    value in closed form, and a random field against a brute-force evaluation of
+    its Fourier-Chebyshev sums at every grid point.
 """
 import json
 import os
@@ -230,3 +233,56 @@ def test_field_io_and_generator():
     p = R.field_physical(g, u)
     assert np.max(np.abs(p[:, 0])) < 1e-15 and np.max(np.abs(p[:, -1])) < 1e-15
     assert np.max(np.abs(R.field_spectral(g, p) - u)) < 1e-15
+
+
+def _physical_bruteforce(u, Nx, Ny, Nz, Lx, Lz):
+    """Values of a spectral field at every grid point by direct sums:
+    u(x,y,z) = Re sum_{kx,kz} w_kz sum_n u_n(kx,kz) T_n(y) exp(2 pi i (kx x/Lx + kz z/Lz)),
+    w_0 = 1, w_kz = 2 (kz > 0), y_j = cos(pi j/(Ny-1)), x = Lx i/Nx, z = Lz k/Nz."""
+    y = np.cos(np.pi * np.arange(Ny) / (Ny - 1))
+    T = np.cos(np.outer(np.arange(Ny), np.arccos(np.clip(y, -1, 1))))        # T[n, j] = T_n(y_j)
+    kx = np.fft.fftfreq(Nx, 1.0 / Nx)
+    kz = np.arange(Nz // 2 + 1)
+    wz = np.where(kz == 0, 1.0, 2.0)
+    ex = np.exp(2j * np.pi * np.outer(np.arange(Nx), kx) / Nx)              # [x, mx]
+    ez = np.exp(2j * np.pi * np.outer(np.arange(Nz), kz) / Nz) * wz         # [z, mz]
+    out = np.zeros((u.shape[0], Ny, Nx, Nz))
+    for c in range(u.shape[0]):
+        uy = np.einsum("nj,nam->jam", T, u[c])                              # [y, mx, mz]
+        out[c] = np.einsum("xa,jam,zm->jxz", ex, uy, ez).real
+    return out
+
+
+def test_cfl_closed_form_and_bruteforce():
+    Nx, Ny, Nz = 16, 17, 12
+    g = R.Grid(Nx, Ny, Nz, LX, LZ)
+    fl = dict(baseflow="LaminarBase", ulowerwall=-1.0, uupperwall=1.0, constraint="BulkVelocity", Ubulk=0.0,
+              nu=1 / 400, dt=0.02, nonlinearity="SkewSymmetric", dealiasing="DealiasXZ")
+    # laminar Couette U = y: the largest (u + U)/dx is 1/(Lx/Nx) at the upper wall, v = w = 0
+    d = R.OracleDNS(g, R.make_flags(**fl), g.zeros(3), g.zeros(1))
+    assert d.cfl() == pytest.approx(Nx / LX * 2 * np.pi / 3 * 0.02, rel=1e-14)
+    # a random field plus U = y, against direct sums; no base flow without dealiasing (factor pi dt)
+    u = random_field(Nx, Ny, Nz, LX, LZ, norm=0.3, decay=0.5, seed=11)
+    y = np.cos(np.pi * np.arange(Ny) / (Ny - 1))
+    dy = np.empty(Ny)
+    dy[0] = dy[-1] = y[0] - y[1]
+    dy[1:-1] = (y[:-2] - y[2:]) / 2
+    # one velocity component kept at a time, so that each term decides the maximum once, and U = 8y,
+    # which dominates; the CFL does not need a divergence-free field
+    cases = []
+    for keep in (0, 1, 2):
+        uk = np.zeros_like(u)
+        uk[keep] = u[keep]
+        cases.append((f"component {keep} only, ZeroBase", uk, dict(fl, baseflow="ZeroBase"), 0.0 * y))
+    cases.append(("all components, U = 8y", u, dict(fl, ulowerwall=-8.0, uupperwall=8.0), 8.0 * y))
+    winners = set()
+    for name, uk, fl2, U in cases:
+        phys = _physical_bruteforce(uk, Nx, Ny, Nz, LX, LZ)
+        terms = [np.max((phys[0] + U[:, None, None]) / (LX / Nx)), np.max(phys[1] / dy[:, None, None]),
+                 np.max(phys[2] / (LZ / Nz))]
+        winners.add(int(np.argmax(terms)))
+        ref = max(max(terms), 0.0) * 2 * np.pi / 3 * 0.02
+        d = R.OracleDNS(g, R.make_flags(**fl2), uk, g.zeros(1))
+        print(f"{name}: oracle CFL {d.cfl():.15e}, direct sums {ref:.15e}")
+        assert d.cfl() == pytest.approx(ref, rel=1e-12)
+    assert winners == {0, 1, 2}
