@@ -75,6 +75,37 @@ __device__ __forceinline__ void dft_small(double2 (&v)[4]) {
     for (int r = 0; r < R; ++r) v[r] = o[r];
 }
 
+// radix 8, decimation in frequency: sums and differences of v_k, v_{k+4}, the
+// differences turned by w^k (w = exp(SIGN 2 pi i/8)), then two radix-4 DFTs
+// give the even and the odd outputs
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_small(double2 (&v)[8]) {
+    static_assert(R == 8, "radix-8 overload");
+    constexpr double h = 0.70710678118654752440;  // 1/sqrt(2)
+    double2 a[4], b[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        a[k] = cadd(v[k], v[k + 4]);
+        b[k] = csub(v[k], v[k + 4]);
+    }
+    if constexpr (SIGN < 0) {  // w = (1 - i)/sqrt2, w^2 = -i, w^3 = -(1 + i)/sqrt2
+        b[1] = make_double2(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));
+        b[2] = make_double2(b[2].y, -b[2].x);
+        b[3] = make_double2(h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y));
+    } else {  // w = (1 + i)/sqrt2, w^2 = i, w^3 = (-1 + i)/sqrt2
+        b[1] = make_double2(h * (b[1].x - b[1].y), h * (b[1].x + b[1].y));
+        b[2] = make_double2(-b[2].y, b[2].x);
+        b[3] = make_double2(-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y));
+    }
+    dft_small<4, SIGN>(a);
+    dft_small<4, SIGN>(b);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        v[2 * m] = a[m];
+        v[2 * m + 1] = b[m];
+    }
+}
+
 template <int R, int SIGN>
 __device__ __forceinline__ void stockham_bfly(const double2* __restrict__ s, double2* __restrict__ d, const double2* tw,
                                               int j, int k, int NR, int NS, int tstep) {
@@ -121,9 +152,26 @@ __device__ double2* fft_lds(double2* a, double2* b, const FFTPlan& P, const doub
     return src;
 }
 
-// compile-time length: every divisor is a constant
-constexpr int fft_radix(int n, int p) {
+// compile-time length: every divisor is a constant. Radices 4, 2, 3 in that
+// order; with r8, radix 8 first where that takes fewer passes (ties: 4). The
+// z-pass gains from the fewer passes; the x-FFTs, with fewer lines per block,
+// lose more to the smaller number of butterflies per pass.
+constexpr int fft_passes_from(int n, bool eight) {
     int c = 0;
+    while (eight && n % 8 == 0) n /= 8, ++c;
+    while (n % 4 == 0) n /= 4, ++c;
+    while (n % 2 == 0) n /= 2, ++c;
+    while (n % 3 == 0) n /= 3, ++c;
+    return c;
+}
+constexpr int fft_radix(int n, int p, bool r8) {
+    const bool eight = r8 && fft_passes_from(n, true) < fft_passes_from(n, false);
+    int c = 0;
+    while (eight && n % 8 == 0) {
+        if (c == p) return 8;
+        n /= 8;
+        ++c;
+    }
     while (n % 4 == 0) {
         if (c == p) return 4;
         n /= 4;
@@ -141,38 +189,75 @@ constexpr int fft_radix(int n, int p) {
     }
     return 0;
 }
-constexpr int fft_npass(int n) {
+constexpr int fft_npass(int n, bool r8) {
     int c = 0;
-    while (fft_radix(n, c) != 0) ++c;
+    while (fft_radix(n, c, r8) != 0) ++c;
     return c;
 }
-constexpr int fft_ns(int n, int p) {
+constexpr int fft_ns(int n, int p, bool r8) {
     int s = 1;
-    for (int q = 0; q < p; ++q) s *= fft_radix(n, q);
+    for (int q = 0; q < p; ++q) s *= fft_radix(n, q, r8);
     return s;
 }
 
-template <int N, int SIGN, int P>
+// Threads per block of every kernel that runs the compile-time FFTs.
+constexpr int FFT_BLOCK = 256;
+
+// One Stockham pass per recursion level. A thread keeps the same butterfly
+// index j in every line it visits (FFT_BLOCK / NR lines per sweep), so its
+// padded source and destination offsets and its twiddles are computed once
+// per pass; a twiddle of 1 (k = 0) multiplies exactly.
+template <int N, int SIGN, int P, bool R8>
 __device__ __forceinline__ double2* fft_ct(double2* a, double2* b, const double2* tw, int nlines) {
-    if constexpr (P == fft_npass(N)) {
+    if constexpr (P == fft_npass(N, R8)) {
         return a;
     } else {
-        constexpr int R = fft_radix(N, P), NR = N / R, NS = fft_ns(N, P), TSTEP = N / (NS * R);
-        for (int w = threadIdx.x; w < nlines * NR; w += blockDim.x) {
-            const int line = w / NR;
-            const int j = w - line * NR;
-            stockham_bfly<R, SIGN>(a + line * lpitch(N), b + line * lpitch(N), tw, j, j % NS, NR, NS, TSTEP);
+        constexpr int R = fft_radix(N, P, R8), NR = N / R, NS = fft_ns(N, P, R8), TSTEP = N / (NS * R);
+        constexpr int LPI = FFT_BLOCK / NR, LP = lpitch(N);
+        static_assert(LPI >= 1, "FFT line longer than the block");
+        const int j = threadIdx.x % NR, l0 = threadIdx.x / NR;
+        if (l0 < LPI) {
+            const int k = j % NS;
+            const int o = (j / NS) * NS * R + k;
+            int so[R], dof[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                so[r] = pidx(j + r * NR);
+                dof[r] = pidx(o + r * NS);
+            }
+            double2 t[R];
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    t[r] = tw[r * k * TSTEP];
+                    if (SIGN > 0) t[r].y = -t[r].y;
+                }
+            }
+            for (int line = l0; line < nlines; line += LPI) {
+                const double2* __restrict__ src = a + line * LP;
+                double2* __restrict__ dst = b + line * LP;
+                double2 v[R == 8 ? 8 : 4];
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[r] = src[so[r]];
+                if constexpr (NS > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], t[r]);
+                }
+                dft_small<R, SIGN>(v);
+#pragma unroll
+                for (int r = 0; r < R; ++r) dst[dof[r]] = v[r];
+            }
         }
         __syncthreads();
-        return fft_ct<N, SIGN, P + 1>(b, a, tw, nlines);
+        return fft_ct<N, SIGN, P + 1, R8>(b, a, tw, nlines);
     }
 }
 
 // N == 0 selects the runtime-length path
-template <int N, int SIGN>
+template <int N, int SIGN, bool R8 = false>
 __device__ __forceinline__ double2* fft_any(double2* a, double2* b, const FFTPlan& P, const double2* tw, int nlines) {
     if constexpr (N == 0) return fft_lds<SIGN>(a, b, P, tw, nlines);
-    else return fft_ct<N, SIGN, 0>(a, b, tw, nlines);
+    else return fft_ct<N, SIGN, 0, R8>(a, b, tw, nlines);
 }
 
 // stage the twiddle table of P into LDS
@@ -719,7 +804,7 @@ __device__ void zc2r_pair(const ZCtx& c, const double2* __restrict__ xp, int fa,
         }
     }
     __syncthreads();
-    double2* r = fft_any<NZ, +1>(c.wa, c.wb, c.pz, c.tw, c.nl);
+    double2* r = fft_any<NZ, +1, true>(c.wa, c.wb, c.pz, c.tw, c.nl);
     double* pa = physf(c, da);
     double* pb = db >= 0 ? physf(c, db) : nullptr;
     for (int w = threadIdx.x; w < c.nl * Nz; w += blockDim.x) {
@@ -744,7 +829,7 @@ __device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int g
         c.wa[l * P + pidx(zz)] = make_double2(pa[w], pb ? pb[w] : 0.0);
     }
     __syncthreads();
-    double2* r = fft_any<NZ, -1>(c.wa, c.wb, c.pz, c.tw, c.nl);
+    double2* r = fft_any<NZ, -1, true>(c.wa, c.wb, c.pz, c.tw, c.nl);
     const long fsz = (long)g.Ny * g.Nx * nkz;
     for (int w = threadIdx.x; w < c.nl * nkz; w += blockDim.x) {
         const int l = w / nkz, k = w - l * nkz;
