@@ -28,7 +28,7 @@ EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_des
             "cfg_dns_set_state", "cfg_dns_get_state", "cfg_dns_reset_dt", "cfg_dns_reset_time",
             "cfg_dns_reset_gradp", "cfg_dns_reset_bulkv", "cfg_dns_time", "cfg_dns_dpdx", "cfg_dns_ubulk",
             "cfg_dns_cfl", "cfg_dns_base", "cfg_dns_device_state", "cfg_dns_profile", "cfg_dns_kernel_times",
-            "cfg_kernel_name", "cfg_nonlinear", "cfg_solve",
+            "cfg_kernel_name", "cfg_nonlinear", "cfg_solve", "cfg_pressure",
             "cfg_batch_create", "cfg_batch_destroy", "cfg_batch_add", "cfg_batch_remove", "cfg_batch_advance",
             "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_dpdx",
             "cfg_batch_cfl", "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats",
@@ -92,6 +92,7 @@ def library():
         "cfg_solve": ([G, F, d, P, P, P, P, P], i),
         "cfg_batch_create": ([C.POINTER(v)], i),
         "cfg_batch_destroy": ([v], i),
+        "cfg_pressure": ([G, F, P, P, P, P], i),
         "cfg_batch_add": ([v, G, F, P, P, C.POINTER(C.c_int)], i),
         "cfg_batch_remove": ([v, i], i),
         "cfg_batch_advance": ([v, i], i),
@@ -426,6 +427,31 @@ class Batch:
         c, r = C.c_longlong(), C.c_longlong()
         _check(library().cfg_batch_graph_stats(self._h, C.byref(c), C.byref(r)))
         return c.value, r.value
+
+
+class PressureSolver:
+    """PressureSolver(u, Ubase, Wbase, nu, Vsuck, nonlinearity) (poissonsolver.h:63-97),
+    on the device. solve(q, u) fills q with the pressure of u: lapl q = -div N(u+U),
+    dq/dy = nu v'' at the walls (the initial q of simulateflow.cpp:104-105). With
+    Ubase = Wbase = None the base flow is the one a DNS with `flags` builds."""
+
+    def __init__(self, u, Ubase=None, Wbase=None, nu=0.0025, Vsuck=0.0, nonlinearity="Rotational", flags=None):
+        self.grid = u.grid()
+        f = flags if flags is not None else DNSFlags()
+        self.flags = f.copy(nu=nu, Vsuck=Vsuck, nonlinearity=nonlinearity) if flags is None else f
+        Ny = self.grid.Ny
+        self.U = None if Ubase is None else np.ascontiguousarray(np.asarray(Ubase, dtype=np.float64)[:Ny])
+        self.W = None if Wbase is None else np.ascontiguousarray(np.asarray(Wbase, dtype=np.float64)[:Ny])
+        if self.U is not None and self.W is None:
+            self.W = np.zeros(Ny)
+
+    def solve(self, q, u):
+        uu = _f64(u.data)
+        out = np.zeros(q.data.shape, dtype=np.complex128)
+        _check(library().cfg_pressure(C.byref(self.grid), C.byref(self.flags.c()), _ptr(self.U), _ptr(self.W),
+                                      _ptr(uu), _ptr(out.view(np.float64))))
+        q.data[...] = out
+        return q
 
 
 class Server:

@@ -556,6 +556,122 @@ __global__ void __launch_bounds__(W * MPB) k_tau_solve_wave(Geom g, const TauMem
     }
 }
 
+// PressureSolver::solve (poissonsolver.cpp:326-431) per mode, after the
+// nonlinear term f of u + U: p'' - kappa^2 p = -div f with p = 0 at the walls,
+// solved with the pressure factors of the tau data (the same Helmholtz operator,
+// nu = 1), then for every mode but (0,0) the wall correction
+// g = c exp(-mu (y-a)) + d exp(mu (y-b)), mu^2 = kappa^2, that makes
+// dp/dy = nu v'' at y = a, b; g's coefficients are Ti times its values at the
+// Chebyshev points.
+template <int K, int W, int MPB>
+__global__ void __launch_bounds__(W * MPB) k_pressure_wave(Geom g, YMats ym, PressArgs pa) {
+    extern __shared__ double2 lds[];  // [MPB][4][Ny]: f0, f1, f2, v
+    const int Ny = g.Ny;
+    const long M = g.M;
+    const int m0 = blockIdx.x * MPB, w = threadIdx.x / W;
+    const long m = m0 + w;
+    const bool valid = m < M;
+    const double2* __restrict__ f = reinterpret_cast<const double2*>(pa.f);
+    const double2* __restrict__ u = reinterpret_cast<const double2*>(pa.u);
+    double2* __restrict__ q = reinterpret_cast<double2*>(pa.q);
+    for (int idx = threadIdx.x; idx < 4 * Ny * MPB; idx += W * MPB) {
+        const int ww = idx % MPB, rest = idx / MPB, n = rest % Ny, comp = rest / Ny;
+        const long mm = m0 + ww;
+        double2 v = make_double2(0.0, 0.0);
+        if (mm < M) v = comp < 3 ? f[((long)comp * Ny + n) * M + mm] : u[((long)Ny + n) * M + mm];
+        lds[(ww * 4 + comp) * Ny + n] = v;
+    }
+    __syncthreads();
+    WCtx c;
+    c.N = Ny;
+    c.Me = (Ny + 1) / 2;
+    c.Mo = (Ny - 1) / 2;
+    c.Nb = Ny - 1;
+    c.Nc = Ny - 1;
+    c.lane = threadIdx.x % W;
+    c.L = g.b - g.a;
+    if (valid) {
+        const int ikx = (int)(m % g.nkx), ikz = (int)(m / g.nkx);
+        const int kx = kx_of(g, ikx), kz = ikz;
+        const bool zero = (kx == 0 && kz == 0);
+        const double pi2 = PI * PI;
+        const double kappa2 = 4 * pi2 * ((kx / g.Lx) * (kx / g.Lx) + (kz / g.Lz) * (kz / g.Lz));
+        const double ax = 2 * PI * kx / g.Lx, az = 2 * PI * kz / g.Lz;
+        const double* cm = pa.set.coef + m * TAU_NA * Ny;
+        WV<K> Fxr, Fxi, Fyr, Fyi, Fzr, Fzi;
+        wload_c<K, W>(lds + (w * 4 + 0) * Ny, c, Fxr, Fxi);
+        wload_c<K, W>(lds + (w * 4 + 1) * Ny, c, Fyr, Fyi);
+        wload_c<K, W>(lds + (w * 4 + 2) * Ny, c, Fzr, Fzi);
+        const WV<K> Dr = wdiff<K, W>(Fyr, c), Di = wdiff<K, W>(Fyi, c);
+        WV<K> Rr, Ri;  // -div f (diffops.cpp:2488-2516)
+        for (int k = 0; k < K; ++k) {
+            Rr.e[k] = -((-ax * Fxi.e[k] - az * Fzi.e[k]) + Dr.e[k]);
+            Rr.o[k] = -((-ax * Fxi.o[k] - az * Fzi.o[k]) + Dr.o[k]);
+            Ri.e[k] = -((ax * Fxr.e[k] + az * Fzr.e[k]) + Di.e[k]);
+            Ri.o[k] = -((ax * Fxr.o[k] + az * Fzr.o[k]) + Di.o[k]);
+        }
+        WV<K> Pr = whelm<K, W>(Rr, cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
+        WV<K> Pi = whelm<K, W>(Ri, cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
+        if (!zero) {  // dp/dy = nu v'' at the walls (poissonsolver.cpp:356-427)
+            WV<K> Vr, Vi;
+            wload_c<K, W>(lds + (w * 4 + 3) * Ny, c, Vr, Vi);
+            double pbr, par_, pbi, pai, vbr, var_, vbi, vai;
+            wends<K, W>(wdiff<K, W>(Pr, c), c, pbr, par_);
+            wends<K, W>(wdiff<K, W>(Pi, c), c, pbi, pai);
+            wends<K, W>(wdiff<K, W>(wdiff<K, W>(Vr, c), c), c, vbr, var_);
+            wends<K, W>(wdiff<K, W>(wdiff<K, W>(Vi, c), c), c, vbi, vai);
+            const double alr = pa.nu * var_ - par_, ali = pa.nu * vai - pai;
+            const double ber = pa.nu * vbr - pbr, bei = pa.nu * vbi - pbi;
+            const double mu = sqrt(kappa2), H = g.b - g.a, e = exp(-mu * H);
+            const double delta = mu * (1 - exp(-2 * mu * H));
+            const double ccr = (-alr + ber * e) / delta, cci = (-ali + bei * e) / delta;
+            const double ddr = (ber - alr * e) / delta, ddi = (bei - ali * e) / delta;
+            const double piN = PI / (Ny - 1), rad = (g.b - g.a) / 2, cen = (g.b + g.a) / 2;  // chebypoints
+            for (int par = 0; par < 2; ++par)
+                for (int k = 0; k < K; ++k) {
+                    const int n = 2 * (K * c.lane + k) + par;
+                    if (n >= Ny) continue;
+                    double sr = 0.0, si = 0.0;
+                    for (int j = 0; j < Ny; ++j) {
+                        const double y = cen + rad * cos(j * piN);
+                        const double e1 = exp(-mu * (y - g.a)), e2 = exp(mu * (y - g.b));
+                        const double t = ym.Ti[(long)n * Ny + j];
+                        sr += t * (ccr * e1 + ddr * e2);
+                        si += t * (cci * e1 + ddi * e2);
+                    }
+                    if (par == 0) {
+                        Pr.e[k] += sr;
+                        Pi.e[k] += si;
+                    } else {
+                        Pr.o[k] += sr;
+                        Pi.o[k] += si;
+                    }
+                }
+        }
+        wstore_c<K, W>(lds + (w * 4 + 0) * Ny, c, Pr, Pi);
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < Ny * MPB; idx += W * MPB) {
+        const int ww = idx % MPB, n = idx / MPB;
+        const long mm = m0 + ww;
+        if (mm < M) q[(long)n * M + mm] = lds[(ww * 4 + 0) * Ny + n];
+    }
+}
+
+void launch_pressure(const Geom& g, const YMats& ym, const PressArgs& pa, hipStream_t st) {
+    const int Me = (g.Ny + 1) / 2;
+    auto go = [&](auto kern, int w) {
+        const int mpb = 256 / w;
+        const dim3 grid((unsigned)((g.M + mpb - 1) / mpb));
+        kern<<<grid, 256, (size_t)4 * g.Ny * mpb * sizeof(double2), st>>>(g, ym, pa);
+    };
+    if (Me <= 16) go(k_pressure_wave<1, 16, 16>, 16);
+    else if (Me <= 32) go(k_pressure_wave<1, 32, 8>, 32);
+    else if (Me <= 64) go(k_pressure_wave<1, 64, 4>, 64);
+    else if (Me <= 128) go(k_pressure_wave<2, 64, 4>, 64);
+    else go(k_pressure_wave<3, 64, 4>, 64);
+}
+
 void launch_tau_solve(const Geom& g, int n, const TauMember* tab, hipStream_t st) {
     const int Me = (g.Ny + 1) / 2;  // even coefficients: at most K per lane of a W-lane segment
     // 256-thread blocks: several per CU, so that one block's HBM staging overlaps

@@ -174,6 +174,17 @@ struct NLMember {
 
 // ---- tau ------------------------------------------------------------------
 void launch_tau_setup(const Geom& g, const TauSet& s, double nu, bool taucorr, double* scratch, hipStream_t st);
+// PressureSolver::solve (poissonsolver.cpp:326-431) of one field: q from the
+// nonlinear term f of u + U and from v; the pressure factors come from set.
+struct PressArgs {
+    const double* f;  // compact 3-component nonlinear term
+    const double* u;  // compact velocity (v is used)
+    double* q;        // compact pressure out
+    TauSet set;
+    double nu;
+};
+void launch_pressure(const Geom& g, const YMats& ym, const PressArgs& pa, hipStream_t st);
+
 void launch_tau_solve(const Geom& g, int n, const TauMember* tab, hipStream_t st);
 
 // ---- elementwise ----------------------------------------------------------

@@ -198,6 +198,35 @@ int cfg_nonlinear(const cfg_grid* grid, const cfg_flags* flags, const double* u_
     });
 }
 
+// PressureSolver(u, Ubase, Wbase, nu, Vsuck, nonlinearity).solve(q, u)
+// (poissonsolver.cpp:270-431): the nonlinear term of u + U (rotation off, as
+// the solver's own DNSFlags have it), then the per-mode Poisson solve and wall
+// correction. The pressure factors are those of any tau set on this grid.
+int cfg_pressure(const cfg_grid* grid, const cfg_flags* flags, const double* Ubase, const double* Wbase,
+                 const double* u_spec, double* q_spec) {
+    return guarded([&] {
+        if (!grid || !flags || !u_spec || !q_spec) throw Error(CFG_ERR_ARG, "null argument");
+        check_flags(*flags);
+        check_grid(*grid);
+        cfg_flags fl = *flags;
+        fl.rotation = 0.0;
+        Exec ex(*grid, fl.dealiasing == CFG_DealiasXZ);
+        const Geom& g = ex.geo->g;
+        PBuf u(ex.pool, ex.nfield), f(ex.pool, ex.nfield), q(ex.pool, ex.nscal);
+        std::vector<double> up = upload_packed(g, 3, u_spec, u.p, ex.st);
+        auto model = std::make_shared<FlowModel>(*ex.geo, fl, up, Ubase, Wbase);
+        NSECtx nse(ex, model, fl);
+        TickOps ops;
+        nse.nonlinear(u.p, f.p, ops);
+        ex.run({&ops});
+        auto tau = tau_for(*ex.geo, fl.nu, 1.0, true, ex.st, ex.pool, nullptr);
+        if (tau->ready) HIPCHK(hipStreamWaitEvent(ex.st, tau->ready, 0));
+        launch_pressure(g, ex.geo->ymats(), PressArgs{f.p, u.p, q.p, tau->set, fl.nu}, ex.st);
+        HIPCHK(hipGetLastError());
+        download_packed(g, 1, q.p, ex.nscal, q_spec, ex.st);
+    });
+}
+
 int cfg_solve(const cfg_grid* grid, const cfg_flags* flags, double lambda_t, const double* u0_spec,
               const double* rhs_spec, double* u_spec, double* q_spec, double* dPdx) {
     return guarded([&] {

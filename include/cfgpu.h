@@ -110,6 +110,15 @@ int cfg_nonlinear(const cfg_grid* grid, const cfg_flags* flags, const double* u_
 int cfg_solve(const cfg_grid* grid, const cfg_flags* flags, double lambda_t, const double* u0_spec,
               const double* rhs_spec, double* u_spec, double* q_spec, double* dPdx);
 
+/* PressureSolver(u, Ubase, Wbase, nu, Vsuck, nonlinearity).solve(q, u)
+ * (poissonsolver.h:63-97, poissonsolver.cpp:270-431): q with lapl q = -div N(u+U)
+ * and dq/dy = nu v'' at the walls, the initial pressure simulateflow.cpp:104-105
+ * computes before DNS::advance. nu, Vsuck and nonlinearity come from flags;
+ * Ubase/Wbase are Chebyshev coefficients of length Ny, or NULL for the base flow a
+ * DNS with these flags builds. u and q in the reference layout. */
+int cfg_pressure(const cfg_grid* grid, const cfg_flags* flags, const double* Ubase, const double* Wbase,
+                 const double* u_spec, double* q_spec);
+
 /* ---- Batched integrations ---------------------------------------------------
  * Many independent DNS integrations on one device, of any grids. channelflow's
  * callers run them one after another: edge-tracking bisection members

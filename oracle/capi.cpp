@@ -115,6 +115,23 @@ void cpuref_dns_base(void* hp, double* U, double* W) {
 }
 void cpuref_dns_destroy(void* hp) { delete static_cast<Handle*>(hp); }
 
+// PressureSolver(u, dns.Ubase(), dns.Wbase(), nu, Vsuck, nonlinearity).solve(p, u),
+// as simulateflow.cpp:100-105 calls it: the base flow is the one a DNS with
+// these flags builds.
+int cpuref_pressure(int Nx, int Ny, int Nz, double Lx, double Lz, double a, double b, const cpuref_flags* fl,
+                    const double* u_spec, double* p_spec) {
+    return guard<int>(-1, [&]() -> int {
+            std::vector<FlowField> in = {FlowField(Nx, Ny, Nz, 3, Lx, Lz, a, b), FlowField(Nx, Ny, Nz, 1, Lx, Lz, a, b)};
+            load(in[0], u_spec);
+            const DNSFlags f = toflags(fl);
+            NSE nse(in, f);
+            FlowField p;
+            pressure_solve(in[0], nse.Ubase(), nse.Wbase(), f.nu, f.Vsuck, f.nonlinearity, p);
+            store(p, p_spec);
+            return 0;
+        });
+}
+
 // One NSE::nonlinear call (nse.cpp:383-391): base flow added, NL evaluated,
 // aliased modes of f zeroed. u is round-tripped in place as in the reference.
 int cpuref_nonlinear(int Nx, int Ny, int Nz, double Lx, double Lz, double a, double b, const cpuref_flags* fl,

@@ -1581,4 +1581,87 @@ void DNS::reset_bulkv(Real Ubulk, Real Wbulk) {  // dns.cpp:227-232
     main_nse_->reset_bulkv(Ubulk, Wbulk);
 }
 
+// ============================================================================
+// PressureSolver (poissonsolver.cpp:270-431)
+// ============================================================================
+// p solves lapl p = -div nonl(u + U) with p = 0 at the walls, mode by mode
+// (PoissonSolver::solve, :146-171: a HelmholtzSolver of lambda = 4 pi^2
+// (kx^2/Lx^2 + kz^2/Lz^2), nu = 1, per mode; the divergence as div's 3d branch,
+// diffops.cpp:2488-2516). Every mode but (0,0) then gets g = c exp(-mu (y-a)) +
+// d exp(mu (y-b)), mu^2 = lambda, chosen so that dp/dy = nu v'' at y = a, b
+// (:356-427); g is evaluated at the Chebyshev points and transformed.
+void pressure_solve(const FlowField& u0, const ChebyCoeff& Ubase, const ChebyCoeff& Wbase, Real nu, Real Vsuck,
+                    NonlinearMethod nonl, FlowField& p) {
+    const int Nx = u0.Nx(), Ny = u0.Ny(), Nz = u0.Nz(), Mz = u0.Mz();
+    const Real Lx = u0.Lx(), Lz = u0.Lz(), a = u0.a(), b = u0.b();
+    FlowField u(u0);
+    FlowField f(Nx, Ny, Nz, 3, Lx, Lz, a, b), tmp;
+    DNSFlags fl;
+    fl.nonlinearity = nonl;
+    fl.Vsuck = Vsuck;
+    fl.nu = nu;
+    navierstokesNL(u, Ubase, Wbase, f, tmp, fl);
+    p = FlowField(Nx, Ny, Nz, 1, Lx, Lz, a, b);
+    const int kxmax = f.kxmax(), kzmax = f.kzmax();
+    const std::vector<Real> y = chebypoints(Ny, a, b);
+    const Real H = b - a;
+    ChebyCoeff gr(Ny, a, b), gi(Ny, a, b), gyr(Ny, a, b), gyi(Ny, a, b);
+    for (int mx = 0; mx < Nx; ++mx)
+        for (int mz = 0; mz < Mz; ++mz) {
+            const int kx = f.kx(mx), kz = mz;
+            const Complex cx(0.0, 2.0 * pi * kx / Lx * zero_last_mode(kx, kxmax, 1));
+            const Complex cz(0.0, 2.0 * pi * kz / Lz * zero_last_mode(kz, kzmax, 1));
+            // -div f
+            for (int my = 0; my < Ny; ++my) {
+                gr[my] = f.cmplx(mx, my, mz, 1).real();
+                gi[my] = f.cmplx(mx, my, mz, 1).imag();
+            }
+            diff(gr, gyr);
+            diff(gi, gyi);
+            ChebyCoeff fr(Ny, a, b), fi(Ny, a, b);
+            for (int my = 0; my < Ny; ++my) {
+                Complex d = f.cmplx(mx, my, mz, 0) * cx + f.cmplx(mx, my, mz, 2) * cz;
+                d += Complex(gyr[my], gyi[my]);
+                d *= -1.0;
+                fr[my] = d.real();
+                fi[my] = d.imag();
+            }
+            const Real lambda = 4.0 * pi * pi * ((kx / Lx) * (kx / Lx) + (kz / Lz) * (kz / Lz));
+            const HelmholtzSolver h(Ny, a, b, lambda);
+            ChebyCoeff pr(Ny, a, b), pi_(Ny, a, b);
+            h.solve(pr, fr, 0.0, 0.0);
+            h.solve(pi_, fi, 0.0, 0.0);
+            if (mx != 0 || mz != 0) {  // dp/dy = nu v_yy at the walls
+                ChebyCoeff pyr(Ny, a, b), pyi(Ny, a, b), vr(Ny, a, b), vi(Ny, a, b), vyyr(Ny, a, b), vyyi(Ny, a, b);
+                diff(pr, pyr);
+                diff(pi_, pyi);
+                for (int my = 0; my < Ny; ++my) {
+                    vr[my] = u.cmplx(mx, my, mz, 1).real();
+                    vi[my] = u.cmplx(mx, my, mz, 1).imag();
+                }
+                diff2(vr, vyyr);
+                diff2(vi, vyyi);
+                const Complex alpha = nu * Complex(vyyr.eval_a(), vyyi.eval_a()) - Complex(pyr.eval_a(), pyi.eval_a());
+                const Complex beta = nu * Complex(vyyr.eval_b(), vyyi.eval_b()) - Complex(pyr.eval_b(), pyi.eval_b());
+                const Real mu = std::sqrt(lambda);
+                const Real delta = mu * (1 - std::exp(-2 * mu * H));
+                const Complex c = (-alpha + beta * std::exp(-mu * H)) / delta;
+                const Complex d = (beta - alpha * std::exp(-mu * H)) / delta;
+                ChebyCoeff g_r(Ny, a, b, Physical), g_i(Ny, a, b, Physical);
+                for (int my = 0; my < Ny; ++my) {
+                    const Complex g = c * std::exp(-mu * (y[my] - a)) + d * std::exp(mu * (y[my] - b));
+                    g_r[my] = g.real();
+                    g_i[my] = g.imag();
+                }
+                g_r.makeSpectral();
+                g_i.makeSpectral();
+                for (int my = 0; my < Ny; ++my) {
+                    pr[my] += g_r[my];
+                    pi_[my] += g_i[my];
+                }
+            }
+            for (int my = 0; my < Ny; ++my) p.cmplx(mx, my, mz, 0) = Complex(pr[my], pi_[my]);
+        }
+}
+
 }  // namespace cfref

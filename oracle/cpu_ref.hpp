@@ -277,6 +277,10 @@ void navierstokesNL(FlowField& u, const ChebyCoeff& Ubase, const ChebyCoeff& Wba
 ChebyCoeff laminarProfile(Real nu, MeanConstraint constraint, Real dPdx, Real Ubulk, Real Vsuck, Real a, Real b,
                           Real ua, Real ub, int Ny);  // nse.cpp:823
 
+// PressureSolver(u, U, W, nu, Vsuck, nonl).solve(p, u) (poissonsolver.cpp:270-431)
+void pressure_solve(const FlowField& u, const ChebyCoeff& Ubase, const ChebyCoeff& Wbase, Real nu, Real Vsuck,
+                    NonlinearMethod nonl, FlowField& p);
+
 // ---- NSE (nse.h/.cpp) -------------------------------------------------------
 class NSE {
    public:

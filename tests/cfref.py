@@ -89,6 +89,7 @@ def lib():
             "cpuref_l2norm2": ([i, i, i, i, d, d, d, d, P, i], d),
             "cpuref_l2dist2": ([i, i, i, i, d, d, d, d, P, P, i], d),
             "cpuref_field_from_io": ([i, i, i, i, d, d, d, d, P, P], i),
+            "cpuref_pressure": ([i, i, i, d, d, d, d, F, P, P], i),
             "cpuref_laminar_profile": ([d, i, d, d, d, d, d, d, d, i, P], i),
         }
         for name, (args, res) in sig.items():
@@ -175,6 +176,15 @@ class OracleDNS:
         if getattr(self, "h", None):
             lib().cpuref_dns_destroy(self.h)
             self.h = None
+
+
+def pressure(grid, flags, u):
+    """PressureSolver(u, Ubase, Wbase, nu, Vsuck, nonlinearity).solve(p, u) with the
+    base flow of a DNS built from the same flags (simulateflow.cpp:100-105)."""
+    uf = _as_f64(u)
+    p = np.zeros((1, grid.Ny, grid.Nx, grid.Nz // 2 + 1), dtype=np.complex128)
+    check(lib().cpuref_pressure(*grid.args(), C.byref(flags), ptr(uf), ptr(p.view(np.float64))))
+    return p
 
 
 def nonlinear(grid, flags, u):

@@ -183,6 +183,41 @@ def test_variable_dt_loop(gpu):
     assert abs(dns.time() - ref.time) < 1e-12
 
 
+PRESSURE_CASES = {
+    "couette_conv": dict(C1_FLAGS, nonlinearity="Convection"),
+    "couette_skew": C1_FLAGS,
+    "poiseuille_rot": CFL_CASES["poiseuille_rot_nodealias"][0] | dict(dealiasing="DealiasXZ"),
+    "suction_rot": CFL_CASES["suction_rot"][0],
+}
+
+
+@pytest.mark.parametrize("case", sorted(PRESSURE_CASES))
+def test_pressure_matches_oracle(gpu, case):
+    """PressureSolver(u, Ubase, Wbase, nu, Vsuck, nonlinearity).solve(q, u): the device's
+    q on the live modes <= 1e-12 relative to the oracle's, with the base flow taken
+    from the flags and passed explicitly."""
+    flags = PRESSURE_CASES[case]
+    Nx, Ny, Nz = 24, 25, 24
+    g = R.Grid(Nx, Ny, Nz, LX_C, LZ_C)
+    u0 = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.2, decay=0.4, seed=51)
+    q_ref = R.pressure(g, R.make_flags(**flags), u0)
+    fu = cfa.FlowField(Nx, Ny, Nz, 3, LX_C, LZ_C, data=u0)
+    fq = cfa.FlowField(Nx, Ny, Nz, 1, LX_C, LZ_C)
+    cfa.PressureSolver(fu, flags=cfa.DNSFlags(**flags)).solve(fq, fu)
+    ref = R.OracleDNS(g, R.make_flags(**flags), u0, g.zeros(1))
+    U, W = ref.base()
+    fq2 = cfa.FlowField(Nx, Ny, Nz, 1, LX_C, LZ_C)
+    cfa.PressureSolver(fu, U, W, flags["nu"], flags.get("Vsuck", 0.0), flags["nonlinearity"]).solve(fq2, fu)
+    mask = live_mask(Nx, Nz, Nx // 3 - 1, Nz // 3 - 1)
+    a, b = fq.data[0][:, mask], q_ref[0][:, mask]
+    err = np.linalg.norm(a - b) / np.linalg.norm(b)
+    err2 = np.linalg.norm(fq2.data[0][:, mask] - b) / np.linalg.norm(b)
+    print(f"{case}: pressure rel L2 vs oracle {err:.3e} (explicit base {err2:.3e}), |q| {np.linalg.norm(b):.3e}",
+          flush=True)
+    assert err <= 1e-12 and err2 <= 1e-12
+    assert np.all(fq.data[0][:, ~mask] == 0)
+
+
 def test_golden_trajectory(gpu):
     """timeIntegrationTest: uinit -> 440 SBDF3 steps -> ufinal, L2Dist <= 1e-13 (the reference's own bar)."""
     man = json.load(open(os.path.join(HERE, "golden", "manifest.json")))

@@ -20,6 +20,10 @@ Pins, by reference test:
   * DNS::CFL (dnsalgo.cpp:63-67, flowfield.cpp:4035-4068): the laminar Couette
     value in closed form, and a random field against a brute-force evaluation of
     its Fourier-Chebyshev sums at every grid point.
+  * PressureSolver (poissonsolver.cpp:326-431): the equations that define p,
+    checked with numpy's own Chebyshev calculus -- the tau rows of
+    p'' - kappa^2 p = -div N(u+U) and dp/dy = nu v'' at both walls (p = 0 at the
+    walls for kx = kz = 0) -- on the modes whose wall correction is resolved.
 """
 import json
 import os
@@ -286,3 +290,47 @@ def test_cfl_closed_form_and_bruteforce():
         print(f"{name}: oracle CFL {d.cfl():.15e}, direct sums {ref:.15e}")
         assert d.cfl() == pytest.approx(ref, rel=1e-12)
     assert winners == {0, 1, 2}
+
+
+def test_pressure_solver_equations():
+    from numpy.polynomial import chebyshev as Ch
+    Nx, Ny, Nz = 16, 33, 16
+    g = R.Grid(Nx, Ny, Nz, LX, LZ)
+    u = random_field(Nx, Ny, Nz, LX, LZ, norm=0.2, decay=0.4, seed=21)
+    cases = [dict(baseflow="LaminarBase", ulowerwall=-1.0, uupperwall=1.0, constraint="BulkVelocity", Ubulk=0.0,
+                  nu=1 / 400, nonlinearity="Convection"),
+             dict(baseflow="LaminarBase", constraint="PressureGradient", dPdx=-3 / 2000, nu=1 / 2000,
+                  nonlinearity="Rotational")]
+    kx = np.fft.fftfreq(Nx, 1.0 / Nx)
+
+    def der(c, m=1):  # d^m/dy^m of Chebyshev coefficients on [-1, 1], same length
+        return np.pad(Ch.chebder(c, m), (0, m))
+
+    for fl in cases:
+        flags = R.make_flags(dealiasing="DealiasXZ", **fl)
+        p = R.pressure(g, flags, u)[0]
+        N = R.nonlinear(g, flags, u.copy())[0]      # live modes as navierstokesNL's
+        checked = 0
+        for mx in range(Nx):
+            for mz in range(Nz // 2 + 1):
+                if abs(kx[mx]) > Nx // 3 - 1 or mz > Nz // 3 - 1:
+                    continue
+                a, c = 2 * np.pi * kx[mx] / LX, 2 * np.pi * mz / LZ
+                mu2 = a * a + c * c
+                if mu2 > 36.0:            # wall correction e^{+-mu y} not resolved at Ny = 33
+                    continue
+                pc = p[:, mx, mz]
+                divN = 1j * a * N[0][:, mx, mz] + der(N[1][:, mx, mz]) + 1j * c * N[2][:, mx, mz]
+                res = der(pc, 2)[:Ny - 2] - mu2 * pc[:Ny - 2] + divN[:Ny - 2]
+                scale = max(np.max(np.abs(divN)), mu2 * np.max(np.abs(pc)), 1e-300)
+                assert np.max(np.abs(res)) <= 1e-9 * scale, (fl["nonlinearity"], kx[mx], mz)
+                if mx == 0 and mz == 0:
+                    assert abs(Ch.chebval(1.0, pc)) < 1e-12 * scale and abs(Ch.chebval(-1.0, pc)) < 1e-12 * scale
+                else:
+                    dp, vyy = der(pc), der(u[1][:, mx, mz], 2)
+                    for yw in (-1.0, 1.0):
+                        bc = Ch.chebval(yw, dp) - fl["nu"] * Ch.chebval(yw, vyy)
+                        assert abs(bc) <= 1e-9 * max(abs(Ch.chebval(yw, dp)), 1e-12), (kx[mx], mz, yw, bc)
+                checked += 1
+        print(fl["nonlinearity"], "modes checked", checked)
+        assert checked >= 10
