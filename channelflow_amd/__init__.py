@@ -215,6 +215,21 @@ class FlowField:
     def grid(self):
         return _CGrid(self.Nx, self.Ny, self.Nz, 0, self.Lx, self.Lz, self.a, self.b)
 
+    def save(self, filebase, padded=True):
+        """FlowField::save (flowfield.cpp:2603-2640) to channelflow's binary .ff
+        format; the NetCDF/HDF5 formats are not part of this library."""
+        if filebase.endswith((".nc", ".h5")):
+            raise CfgError("FlowField.save: only the binary .ff format is written by this library")
+        from .ffio import save_ff
+        save_ff(filebase, self.data, self.Nx, self.Ny, self.Nz, self.Lx, self.Lz, self.a, self.b, padded)
+
+    @classmethod
+    def from_file(cls, filebase):
+        """FlowField(filebase) (flowfield.cpp:215-300) from a spectral .ff file."""
+        from .ffio import load_ff
+        data, m = load_ff(filebase)
+        return cls(m["Nx"], m["Ny"], m["Nz"], m["Nd"], m["Lx"], m["Lz"], m["a"], m["b"], data=data)
+
     def congruent(self, f):
         return (self.Nx, self.Ny, self.Nz, self.Lx, self.Lz, self.a, self.b) == \
             (f.Nx, f.Ny, f.Nz, f.Lx, f.Lz, f.a, f.b)
