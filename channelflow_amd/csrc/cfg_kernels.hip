@@ -7,7 +7,9 @@
 //            with d/dy folded into the operator (T, T*D) and the base flow
 //            U(y), W(y), -Vsuck added to the kx=kz=0 mode;
 //   k_xinv   inverse x-FFT per (field, y, kz) line, optional d/dx;
-//   k_zpass  per (y, x) line: c2r z-FFTs, pointwise products, r2c z-FFTs;
+//   k_zwave  per (y, x) line: c2r z-FFTs, pointwise products, r2c z-FFTs,
+//            a wavefront per group of lines (k_zpass: LDS FFTs, block-wide,
+//            for the lengths the wavefront FFT does not cover);
 //   k_xfwd   forward x-FFT, live kx kept, 1/(Nx Nz) scaling;
 //   k_yfwd_t values -> Chebyshev coefficients, with the spectral divergence
 //            of u_i u_j folded in for the divergence/skew-symmetric forms.
@@ -1103,6 +1105,413 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, const
     }
 }
 
+// ---- wave-local z pass -------------------------------------------------------
+// A z-line of length N = N1 N2 lives on G lanes of one wavefront (G the power
+// of two >= N1, N2; 64/G lines per wavefront), in one of two layouts:
+//   spectral: lane j < N1 holds Z[j + N1 r], r < N2;
+//   physical: lane j < N2 holds x[j + N2 m], m < N1.
+// The inverse transform is an N2-point DFT in registers, the twiddles
+// w_N^(j k1), a transpose through the line's own LDS rows and an N1-point DFT
+// in registers; the forward transform runs the same steps the other way. All
+// fields of a line share the physical layout, so the products are lane-local,
+// and a wavefront never waits for another one.
+template <int NZ>
+struct ZWShape {
+    static constexpr int N1 = 0, N2 = 0;
+};
+template <> struct ZWShape<16> { static constexpr int N1 = 4, N2 = 4; };
+template <> struct ZWShape<32> { static constexpr int N1 = 4, N2 = 8; };
+template <> struct ZWShape<48> { static constexpr int N1 = 6, N2 = 8; };
+template <> struct ZWShape<64> { static constexpr int N1 = 8, N2 = 8; };
+template <> struct ZWShape<96> { static constexpr int N1 = 8, N2 = 12; };
+template <> struct ZWShape<128> { static constexpr int N1 = 8, N2 = 16; };
+template <> struct ZWShape<192> { static constexpr int N1 = 12, N2 = 16; };
+template <> struct ZWShape<256> { static constexpr int N1 = 16, N2 = 16; };
+
+constexpr int pow2_at_least(int n) {
+    int p = 1;
+    while (p < n) p *= 2;
+    return p;
+}
+
+template <int NZ>
+struct ZW {
+    static constexpr int N1 = ZWShape<NZ>::N1, N2 = ZWShape<NZ>::N2, N = N1 * N2;
+    static constexpr int NMAX = N1 > N2 ? N1 : N2;
+    static constexpr int G = pow2_at_least(NMAX);  // lanes per line
+    static constexpr int LPW = 64 / G;             // lines per wavefront
+    static constexpr int LBUF = N + NMAX;          // LDS slots per line (transposes, padded rows)
+};
+
+constexpr bool zwave_size(int n) {
+    return n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 192 || n == 256;
+}
+
+// Orders the LDS accesses of one wavefront (its lanes read what other lanes of
+// the same wavefront wrote): a compiler barrier between wavefront-scope fences.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- DFTs held in registers: sum_n v_n exp(SIGN 2 pi i n k / R) ----
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_reg(double2 (&v)[R]);
+
+constexpr int inv_mod(int a, int m) {
+    for (int x = 1; x < m; ++x)
+        if ((a * x) % m == 1) return x;
+    return 1;
+}
+
+// prime-factor (Good-Thomas) DFT of length A B with gcd(A, B) = 1: input
+// n = (B a + A b) mod AB, output k with k = ka (mod A), k = kb (mod B); no twiddles
+template <int A, int B, int SIGN>
+__device__ __forceinline__ void dft_pfa(double2 (&v)[A * B]) {
+    constexpr int R = A * B, BI = inv_mod(B, A), AI = inv_mod(A, B);
+    double2 t[B][A];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        double2 s[A];
+#pragma unroll
+        for (int a = 0; a < A; ++a) s[a] = v[(B * a + A * b) % R];
+        dft_reg<A, SIGN>(s);
+#pragma unroll
+        for (int a = 0; a < A; ++a) t[b][a] = s[a];
+    }
+#pragma unroll
+    for (int ka = 0; ka < A; ++ka) {
+        double2 s[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) s[b] = t[b][ka];
+        dft_reg<B, SIGN>(s);
+#pragma unroll
+        for (int kb = 0; kb < B; ++kb) v[(B * BI * ka + A * AI * kb) % R] = s[kb];
+    }
+}
+
+// exp(i m pi / 8), m <= 9, from cos(pi/8), sin(pi/8) and 1/sqrt(2)
+__device__ __forceinline__ constexpr double2 w16(int m) {
+    constexpr double c1 = 0.92387953251128675613, s1 = 0.38268343236508977173, h = 0.70710678118654752440;
+    return m == 1 ? double2{c1, s1} : m == 2 ? double2{h, h} : m == 3 ? double2{s1, c1} : m == 4 ? double2{0.0, 1.0}
+         : m == 5 ? double2{-s1, c1} : m == 6 ? double2{-h, h} : m == 7 ? double2{-c1, s1} : m == 8 ? double2{-1.0, 0.0}
+         : m == 9 ? double2{-c1, -s1} : double2{1.0, 0.0};
+}
+
+// length 16 as 4 x 4 (Cooley-Tukey, twiddles exp(SIGN 2 pi i n2 k1 / 16))
+template <int SIGN>
+__device__ __forceinline__ void dft_ct44(double2 (&v)[16]) {
+    double2 t[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+        double2 s[4];
+#pragma unroll
+        for (int n1 = 0; n1 < 4; ++n1) s[n1] = v[4 * n1 + n2];
+        dft_small<4, SIGN>(s);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            if (n2 * k1 == 0) {
+                t[n2][k1] = s[k1];
+            } else {
+                double2 w = w16(n2 * k1);
+                if (SIGN < 0) w.y = -w.y;
+                t[n2][k1] = cmul(s[k1], w);
+            }
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+        double2 s[4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) s[n2] = t[n2][k1];
+        dft_small<4, SIGN>(s);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = s[k2];
+    }
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_reg(double2 (&v)[R]) {
+    if constexpr (R == 2 || R == 3 || R == 4) {
+        double2 t[4];
+#pragma unroll
+        for (int i = 0; i < R; ++i) t[i] = v[i];
+        dft_small<R, SIGN>(t);
+#pragma unroll
+        for (int i = 0; i < R; ++i) v[i] = t[i];
+    } else if constexpr (R == 8) {
+        dft_small<8, SIGN>(v);
+    } else if constexpr (R == 6) {
+        dft_pfa<2, 3, SIGN>(v);
+    } else if constexpr (R == 12) {
+        dft_pfa<4, 3, SIGN>(v);
+    } else if constexpr (R == 16) {
+        dft_ct44<SIGN>(v);
+    } else {
+        static_assert(R == 0, "no register DFT of this length");
+    }
+}
+
+// One line's view: its lane in the group, where its data sit, its LDS rows.
+struct ZWCtx {
+    int gi;            // lane within the line's group
+    bool line_ok;      // the line exists (x < Nx)
+    long base, fsz;    // line offset and field size in the [field][y][x][kz] arrays
+    int nkz;
+    double dk;         // 2 pi / Lz
+    double2* T;        // the line's LDS rows
+    const double2* tw; // exp(-2 pi i m / N), m < N, in LDS
+};
+
+// c2r of two real fields at once (fa -> pa, fb -> pb; fb < 0: none), from the
+// live bins kz < nkz of xp; dza / dzb apply d/dz. As in FFTW's c2r, the
+// imaginary part of bin 0 is ignored.
+template <int NZ>
+__device__ __forceinline__ void zw_c2r(const ZWCtx& c, const double2* __restrict__ xp, int fa, bool dza, int fb,
+                                       bool dzb, double (&pa)[ZW<NZ>::N1], double (&pb)[ZW<NZ>::N1]) {
+    constexpr int N1 = ZW<NZ>::N1, N2 = ZW<NZ>::N2, N = ZW<NZ>::N;
+    const int j = c.gi;
+    const bool act = c.line_ok && j < N1;
+    double2 v[N2];
+#pragma unroll
+    for (int r = 0; r < N2; ++r) {
+        const int n = j + N1 * r;
+        const bool lo = n < c.nkz, hi = n > N - c.nkz;
+        const int k = lo ? n : N - n;
+        double2 A = make_double2(0.0, 0.0), B = make_double2(0.0, 0.0);
+        if (act && (lo || hi)) {
+            A = xp[(long)fa * c.fsz + c.base + k];
+            if (fb >= 0) B = xp[(long)fb * c.fsz + c.base + k];
+        }
+        if (dza) A = imul(c.dk * k, A);
+        if (dzb) B = imul(c.dk * k, B);
+        if (n == 0) v[r] = make_double2(A.x, B.x);
+        else if (lo) v[r] = make_double2(A.x - B.y, A.y + B.x);  // A + iB
+        else v[r] = make_double2(A.x + B.y, B.x - A.y);          // conj(A) + i conj(B) (zero in the gap)
+    }
+    dft_reg<N2, +1>(v);
+    const int jt = j < N1 ? j : N1 - 1;
+#pragma unroll
+    for (int k1 = 1; k1 < N2; ++k1) {
+        double2 t = c.tw[jt * k1];
+        t.y = -t.y;
+        v[k1] = cmul(v[k1], t);
+    }
+    wave_lds_sync();
+    if (j < N1) {
+#pragma unroll
+        for (int k1 = 0; k1 < N2; ++k1) c.T[k1 * (N1 + 1) + j] = v[k1];
+    }
+    wave_lds_sync();
+    const int row = j < N2 ? j : N2 - 1;
+    double2 u[N1];
+#pragma unroll
+    for (int i = 0; i < N1; ++i) u[i] = c.T[row * (N1 + 1) + i];
+    dft_reg<N1, +1>(u);
+#pragma unroll
+    for (int m = 0; m < N1; ++m) {
+        pa[m] = u[m].x;
+        pb[m] = u[m].y;
+    }
+}
+
+// r2c of pa (and pb when TWO) -> zo fields ga (and gb), bins kz < nkz
+template <int NZ, bool TWO>
+__device__ __forceinline__ void zw_r2c(const ZWCtx& c, double2* __restrict__ zo, const double (&pa)[ZW<NZ>::N1],
+                                       const double (&pb)[ZW<NZ>::N1], int ga, int gb) {
+    constexpr int N1 = ZW<NZ>::N1, N2 = ZW<NZ>::N2, N = ZW<NZ>::N;
+    const int j = c.gi;
+    double2 v[N1];
+#pragma unroll
+    for (int m = 0; m < N1; ++m) v[m] = make_double2(pa[m], TWO ? pb[m] : 0.0);
+    dft_reg<N1, -1>(v);
+    const int jt = j < N2 ? j : N2 - 1;
+#pragma unroll
+    for (int q = 1; q < N1; ++q) v[q] = cmul(v[q], c.tw[jt * q]);
+    wave_lds_sync();
+    if (j < N2) {
+#pragma unroll
+        for (int q = 0; q < N1; ++q) c.T[q * (N2 + 1) + j] = v[q];
+    }
+    wave_lds_sync();
+    const int row = j < N1 ? j : N1 - 1;
+    double2 w[N2];
+#pragma unroll
+    for (int i = 0; i < N2; ++i) w[i] = c.T[row * (N2 + 1) + i];
+    dft_reg<N2, -1>(w);  // w[p] = X[row + N1 p]
+    wave_lds_sync();
+    if (j < N1) {
+#pragma unroll
+        for (int p = 0; p < N2; ++p) c.T[j + N1 * p] = w[p];
+    }
+    wave_lds_sync();
+    if (c.line_ok && j < N1) {
+#pragma unroll
+        for (int p = 0; p < N2; ++p) {
+            const int k = j + N1 * p;
+            if (k < c.nkz) {
+                const double2 Zk = w[p];
+                const double2 Zn = c.T[(N - k) % N];
+                zo[(long)ga * c.fsz + c.base + k] = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
+                if (TWO) zo[(long)gb * c.fsz + c.base + k] = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
+            }
+        }
+    }
+}
+
+// The z pass of k_zpass with one wavefront per 64/G lines (same fields, same
+// products in the same order). Block: 4 independent wavefronts.
+template <int METHOD, int NZ>
+__global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict__ twg, const NLMember* __restrict__ tab,
+                                               int want_g, const double2* __restrict__ xp0, double2* __restrict__ zo0,
+                                               long sx) {
+    using S = ZW<NZ>;
+    constexpr int N1 = S::N1;
+    extern __shared__ double2 lds[];
+    for (int i = threadIdx.x; i < NZ; i += blockDim.x) lds[i] = twg[i];
+    __syncthreads();
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int nxc = (g.Nx + S::LPW - 1) / S::LPW;
+    const int wg = blockIdx.x * 4 + wave;
+    const int y = wg / nxc;
+    if (y >= g.Ny) return;
+    const int L = lane / S::G;
+    const int x = (wg - y * nxc) * S::LPW + L;
+    ZWCtx c;
+    c.gi = lane % S::G;
+    c.line_ok = x < g.Nx;
+    c.base = ((long)y * g.Nx + (c.line_ok ? x : 0)) * g.nkz;
+    c.fsz = (long)g.Ny * g.Nx * g.nkz;
+    c.nkz = g.nkz;
+    c.dk = 2 * PI / g.Lz;
+    c.T = lds + NZ + (wave * S::LPW + L) * S::LBUF;
+    c.tw = lds;
+    const double rotation = tab[blockIdx.y].rotation;
+    const double2* __restrict__ xp = xp0 + blockIdx.y * sx;
+    double2* __restrict__ zo = zo0 + blockIdx.y * sx;
+    if constexpr (METHOD == NL_ROT) {
+        double u0[N1], u1[N1], u2[N1], w0[N1], w1[N1], w2[N1];
+        zw_c2r<NZ>(c, xp, 0, false, 1, false, u0, u1);
+        zw_c2r<NZ>(c, xp, 2, false, 3, false, u2, w0);
+        zw_c2r<NZ>(c, xp, 4, false, 5, false, w1, w2);
+        double f0[N1], f1[N1], f2[N1];
+#pragma unroll
+        for (int m = 0; m < N1; ++m) {  // cross(vort, u) (diffops.cpp:2598-2605)
+            const double a0 = u0[m], a1 = u1[m], a2 = u2[m], b0 = w0[m], b1 = w1[m], b2 = w2[m];
+            f0[m] = b1 * a2 - b2 * a1;
+            f1[m] = b2 * a0 - b0 * a2;
+            f2[m] = b0 * a1 - b1 * a0;
+            if (rotation != 0.0) {  // nse.cpp:61-80
+                f0[m] -= rotation * a1;
+                f1[m] += rotation * a0;
+            }
+        }
+        zw_r2c<NZ, true>(c, zo, f0, f1, 0, 1);
+        zw_r2c<NZ, false>(c, zo, f2, f2, 2, -1);
+    } else if constexpr (METHOD == NL_CONV || METHOD == NL_SKEW) {
+        // f_i = sum_j (s u_j) du_i/dx_j, terms added in the order j = x, y, z
+        // (diffops.cpp:3192-3199, 3626-3633). The skew-symmetric form keeps
+        // h = u/2 only: halving is exact, so s u_j is h_j, u_j is 2 h_j and
+        // u_i u_j is 4 (h_i h_j), all with the bits of the unscaled products.
+        constexpr bool SK = (METHOD == NL_SKEW);
+        constexpr double s = SK ? 0.5 : 1.0, us = SK ? 2.0 : 1.0;
+        double h0[N1], h1[N1], h2[N1], pa[N1], pb[N1], fa[N1], fb[N1];
+        zw_c2r<NZ>(c, xp, 0, false, 1, false, h0, h1);   // u, v
+        zw_c2r<NZ>(c, xp, 2, false, 6, false, h2, pa);   // w, du/dx
+        if constexpr (SK) {
+#pragma unroll
+            for (int m = 0; m < N1; ++m) {
+                h0[m] *= s;
+                h1[m] *= s;
+                h2[m] *= s;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < N1; ++m) fa[m] = 0.0 + h0[m] * pa[m];
+        zw_c2r<NZ>(c, xp, 3, false, 0, true, pa, pb);    // du/dy, du/dz
+#pragma unroll
+        for (int m = 0; m < N1; ++m) {
+            fa[m] += h1[m] * pa[m];
+            fa[m] += h2[m] * pb[m];
+            if (rotation != 0.0) fa[m] -= rotation * (us * h1[m]);  // nse.cpp:61-80
+            if (SK) pb[m] = 4.0 * (h0[m] * h0[m]);
+        }
+        zw_r2c<NZ, SK>(c, zo, fa, pb, 0, 3);              // f0 (and uu00)
+        zw_c2r<NZ>(c, xp, 7, false, 4, false, pa, pb);   // dv/dx, dv/dy
+#pragma unroll
+        for (int m = 0; m < N1; ++m) {
+            fa[m] = 0.0 + h0[m] * pa[m];
+            fa[m] += h1[m] * pb[m];
+        }
+        zw_c2r<NZ>(c, xp, 1, true, 8, false, pa, pb);    // dv/dz, dw/dx
+#pragma unroll
+        for (int m = 0; m < N1; ++m) {
+            fa[m] += h2[m] * pa[m];
+            if (rotation != 0.0) fa[m] += rotation * (us * h0[m]);
+            fb[m] = 0.0 + h0[m] * pb[m];
+        }
+        zw_c2r<NZ>(c, xp, 5, false, 2, true, pa, pb);    // dw/dy, dw/dz
+#pragma unroll
+        for (int m = 0; m < N1; ++m) {
+            fb[m] += h1[m] * pa[m];
+            fb[m] += h2[m] * pb[m];
+        }
+        zw_r2c<NZ, true>(c, zo, fa, fb, 1, 2);            // f1, f2
+        if constexpr (SK) {  // remaining entries of outer(u, u) (diffops.cpp:2336-2388)
+#pragma unroll
+            for (int m = 0; m < N1; ++m) {
+                pa[m] = 4.0 * (h0[m] * h1[m]);
+                pb[m] = 4.0 * (h0[m] * h2[m]);
+            }
+            zw_r2c<NZ, true>(c, zo, pa, pb, 4, 5);
+#pragma unroll
+            for (int m = 0; m < N1; ++m) {
+                pa[m] = 4.0 * (h1[m] * h1[m]);
+                pb[m] = 4.0 * (h1[m] * h2[m]);
+            }
+            zw_r2c<NZ, true>(c, zo, pa, pb, 6, 7);
+#pragma unroll
+            for (int m = 0; m < N1; ++m) pa[m] = 4.0 * (h2[m] * h2[m]);
+            zw_r2c<NZ, false>(c, zo, pa, pa, 8, -1);
+        }
+    } else {
+        double u0[N1], u1[N1], u2[N1], pa[N1], pb[N1];
+        zw_c2r<NZ>(c, xp, 0, false, 1, false, u0, u1);
+        zw_c2r<NZ>(c, xp, 2, false, -1, false, u2, pa);
+        if (want_g) {  // DIV with rotation: g = rotation terms only
+#pragma unroll
+            for (int m = 0; m < N1; ++m) {
+                pa[m] = -(rotation * u1[m]);
+                pb[m] = rotation * u0[m];
+            }
+            zw_r2c<NZ, true>(c, zo, pa, pb, 0, 1);
+#pragma unroll
+            for (int m = 0; m < N1; ++m) pa[m] = 0.0;
+            zw_r2c<NZ, false>(c, zo, pa, pa, 2, -1);
+        }
+        // outer(u, u) (diffops.cpp:2336-2388), two entries per r2c
+#pragma unroll
+        for (int m = 0; m < N1; ++m) {
+            pa[m] = u0[m] * u0[m];
+            pb[m] = u0[m] * u1[m];
+        }
+        zw_r2c<NZ, true>(c, zo, pa, pb, 3, 4);
+#pragma unroll
+        for (int m = 0; m < N1; ++m) {
+            pa[m] = u0[m] * u2[m];
+            pb[m] = u1[m] * u1[m];
+        }
+        zw_r2c<NZ, true>(c, zo, pa, pb, 5, 6);
+#pragma unroll
+        for (int m = 0; m < N1; ++m) {
+            pa[m] = u1[m] * u2[m];
+            pb[m] = u2[m] * u2[m];
+        }
+        zw_r2c<NZ, true>(c, zo, pa, pb, 7, 8);
+    }
+}
+
 // ---- y transforms as LDS-tiled contractions ----------------------------------
 // Both directions are dense Ny x Ny products over the Chebyshev index applied to
 // every (component, mode) column. A block computes YB_M modes x YB_R output
@@ -1785,6 +2194,15 @@ static void launch_xfft(bool inverse, const Geom& g, const FFTPlan& px, int nsrc
 template <int METHOD, int NZ>
 static void launch_z_t(const Geom& g, const FFTPlan& pz, int n, const NLMember* tab, int want_g, const double2* xp,
                        double2* zo, long sx, const NLTune& tn, hipStream_t st) {
+    if constexpr (zwave_size(NZ)) {
+        using S = ZW<NZ>;
+        const long waves = (long)g.Ny * ((g.Nx + S::LPW - 1) / S::LPW);
+        const size_t lds = (size_t)(NZ + 4 * S::LPW * S::LBUF) * sizeof(double2);
+        const dim3 grid((unsigned)((waves + 3) / 4), n);
+        allow_lds(k_zwave<METHOD, NZ>, lds);
+        k_zwave<METHOD, NZ><<<grid, 256, lds, st>>>(g, pz.tw, tab, want_g, xp, zo, sx);
+        return;
+    }
     const int nphys = zpass_nphys(METHOD);
     const int XB = zpass_xb(g, nphys, tn);
     const size_t lds = (size_t)g.Nz * sizeof(double2) +
@@ -1943,7 +2361,8 @@ NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     };
     NLTune out;
     float bz = 1e30f;
-    for (int xb = 1; xb <= zpass_xb_max(g, nphys); xb *= 2) {
+    const int xb_max = zwave_size(g.Nz) ? 0 : zpass_xb_max(g, nphys);  // the wavefront z pass has no tiling
+    for (int xb = 1; xb <= xb_max; xb *= 2) {
         NLTune c;
         c.zxb = xb;
         const float ms = timed([&] { launch_z_any(method, g, pz, n, tab, want_g, xp, zo, sx, c, st); });
