@@ -220,6 +220,7 @@ struct Geometry {
     int dealias = 0;
     FFTTable fx, fz;
     DBuf T, TD, Ti, DTi;
+    DBuf HB;  // rows of the Chebyshev-tau operator B of the Helmholtz systems, [3][Ny]
 
     YMats ymats() const { return YMats{T.p, TD.p, Ti.p, DTi.p}; }
 
@@ -275,6 +276,20 @@ struct Geometry {
         up(TD, hTD);
         up(Ti, hTi);
         up(DTi, hDTi);
+        // B of nu u'' - lambda u = f on row n >= 2 (helmholtz.cpp:79-95 with the
+        // c_n and beta_n of helmholtz.h:65-66): c_{n-2}/(4n(n-1)) f[n-2]
+        // - beta_n/(2(n^2-1)) f[n] + beta_{n+2}/(4n(n+1)) f[n+2], the last
+        // term only where row n + 2 belongs to the same parity system
+        std::vector<double> hb(3 * N, 0.0);
+        const int Nc = N - 1;
+        auto cn = [&](int n) { return (n == 0 || n == Nc) ? 2 : 1; };
+        auto beta = [&](int n) { return (n > Nc - 2) ? 0 : 1; };
+        for (int n = 2; n < N; ++n) {
+            hb[n] = ((double)cn(n - 2)) / (4 * n * (n - 1));
+            hb[N + n] = -((double)beta(n)) / (2 * (n * n - 1));
+            hb[2 * N + n] = (n + 2 < N && beta(n + 2)) ? 1.0 / (4 * n * (n + 1)) : 0.0;
+        }
+        up(HB, hb);
     }
 
     // y values of a coefficient profile and of its derivative (long double sums)

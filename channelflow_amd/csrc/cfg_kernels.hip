@@ -388,9 +388,15 @@ __device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, const double* c
 // Runge-Kutta combination, dnsalgo.cpp:218-224, 668-673) with full-line loads
 // into LDS as [mode][c][n], each wave solves its mode in registers, and u, v,
 // w, q leave through LDS the same way.
+// waves per SIMD a tau-solve variant is compiled for: two 16-lane coefficient
+// blocks fit in the 168 registers of three waves
+constexpr int tau_waves(int K, int W) { return (K == 2 && W == 16) ? 3 : 1; }
+
 template <int K, int W, int MPB>
-__global__ void __launch_bounds__(W * MPB) k_tau_solve_wave(Geom g, const TauMember* __restrict__ tab) {
-    extern __shared__ double2 lds[];
+__global__ void __launch_bounds__(W * MPB) __attribute__((amdgpu_waves_per_eu(tau_waves(K, W))))
+k_tau_solve_wave(Geom g, const double* __restrict__ hb,
+                                                             const TauMember* __restrict__ tab) {
+    extern __shared__ double2 lds[];  // [MPB][3][Ny] right-hand sides, then the B rows [3][Ny]
     const TauMember& me = tab[blockIdx.y];
     const TauSet s = me.set;
     const double nu = me.nu;
@@ -405,6 +411,8 @@ __global__ void __launch_bounds__(W * MPB) k_tau_solve_wave(Geom g, const TauMem
     const int w = threadIdx.x / W;  // mode slot of this lane's segment
     const int m = m0 + w;
     const bool valid = m < g.M;
+    double* hbl = reinterpret_cast<double*>(lds + 3 * Ny * MPB);
+    for (int i = threadIdx.x; i < 3 * Ny; i += nthr) hbl[i] = hb[i];
     for (int idx = threadIdx.x; idx < 3 * Ny * MPB; idx += nthr) {
         const int ww = idx % MPB, rest = idx / MPB, n = rest % Ny, comp = rest / Ny;
         const int mm = m0 + ww;
@@ -432,6 +440,7 @@ __global__ void __launch_bounds__(W * MPB) k_tau_solve_wave(Geom g, const TauMem
     c.Nc = Ny - 1;
     c.lane = threadIdx.x % W;
     c.L = g.b - g.a;
+    c.hb = hbl;
     const int ikx = m % g.nkx, ikz = m / g.nkx;
     const int kx = kx_of(g, ikx), kz = ikz;
     const bool zero = (kx == 0 && kz == 0);
@@ -566,9 +575,11 @@ __global__ void __launch_bounds__(W * MPB) k_tau_solve_wave(Geom g, const TauMem
 // dp/dy = nu v'' at y = a, b; g's coefficients are Ti times its values at the
 // Chebyshev points.
 template <int K, int W, int MPB>
-__global__ void __launch_bounds__(W * MPB) k_pressure_wave(Geom g, YMats ym, PressArgs pa) {
-    extern __shared__ double2 lds[];  // [MPB][4][Ny]: f0, f1, f2, v
+__global__ void __launch_bounds__(W * MPB) k_pressure_wave(Geom g, YMats ym, PressArgs pa, const double* __restrict__ hb) {
+    extern __shared__ double2 lds[];  // [MPB][4][Ny]: f0, f1, f2, v; then the B rows [3][Ny]
     const int Ny = g.Ny;
+    double* hbl = reinterpret_cast<double*>(lds + 4 * Ny * MPB);
+    for (int i = threadIdx.x; i < 3 * Ny; i += W * MPB) hbl[i] = hb[i];
     const long M = g.M;
     const int m0 = blockIdx.x * MPB, w = threadIdx.x / W;
     const long m = m0 + w;
@@ -592,6 +603,7 @@ __global__ void __launch_bounds__(W * MPB) k_pressure_wave(Geom g, YMats ym, Pre
     c.Nc = Ny - 1;
     c.lane = threadIdx.x % W;
     c.L = g.b - g.a;
+    c.hb = hbl;
     if (valid) {
         const int ikx = (int)(m % g.nkx), ikz = (int)(m / g.nkx);
         const int kx = kx_of(g, ikx), kz = ikz;
@@ -660,12 +672,13 @@ __global__ void __launch_bounds__(W * MPB) k_pressure_wave(Geom g, YMats ym, Pre
     }
 }
 
-void launch_pressure(const Geom& g, const YMats& ym, const PressArgs& pa, hipStream_t st) {
+void launch_pressure(const Geom& g, const YMats& ym, const double* hb, const PressArgs& pa, hipStream_t st) {
     const int Me = (g.Ny + 1) / 2;
     auto go = [&](auto kern, int w) {
         const int mpb = 256 / w;
         const dim3 grid((unsigned)((g.M + mpb - 1) / mpb));
-        kern<<<grid, 256, (size_t)4 * g.Ny * mpb * sizeof(double2), st>>>(g, ym, pa);
+        const size_t lds = (size_t)4 * g.Ny * mpb * sizeof(double2) + (size_t)3 * g.Ny * sizeof(double);
+        kern<<<grid, 256, lds, st>>>(g, ym, pa, hb);
     };
     if (Me <= 16) go(k_pressure_wave<1, 16, 16>, 16);
     else if (Me <= 32) go(k_pressure_wave<1, 32, 8>, 32);
@@ -674,20 +687,22 @@ void launch_pressure(const Geom& g, const YMats& ym, const PressArgs& pa, hipStr
     else go(k_pressure_wave<3, 64, 4>, 64);
 }
 
-void launch_tau_solve(const Geom& g, int n, const TauMember* tab, hipStream_t st) {
+void launch_tau_solve(const Geom& g, const double* hb, int n, const TauMember* tab, hipStream_t st) {
     const int Me = (g.Ny + 1) / 2;  // even coefficients: at most K per lane of a W-lane segment
     // 256-thread blocks: several per CU, so that one block's HBM staging overlaps
     // another's scans
     auto go = [&](auto kern, int w) {
         const int mpb = 256 / w;
         const dim3 grid((unsigned)((g.M + mpb - 1) / mpb), n);
-        const size_t lds = (size_t)3 * g.Ny * mpb * sizeof(double2);
-        kern<<<grid, 256, lds, st>>>(g, tab);
+        const size_t lds = (size_t)3 * g.Ny * mpb * sizeof(double2) + (size_t)3 * g.Ny * sizeof(double);
+        kern<<<grid, 256, lds, st>>>(g, hb, tab);
     };
-    // one coefficient per lane where it fits (4 waves per SIMD); above 64 even
-    // coefficients two modes per wave at three per lane beat one at two
+    // Up to 48 even coefficients: 16-lane segments (four modes per wave, every
+    // shuffle a DPP row move) with up to three coefficients per lane. Above:
+    // one or two modes per wave, shuffles through LDS permutes.
     if (Me <= 16) go(k_tau_solve_wave<1, 16, 16>, 16);
-    else if (Me <= 32) go(k_tau_solve_wave<1, 32, 8>, 32);
+    else if (Me <= 32) go(k_tau_solve_wave<2, 16, 16>, 16);
+    else if (Me <= 48) go(k_tau_solve_wave<3, 16, 16>, 16);
     else if (Me <= 64) go(k_tau_solve_wave<1, 64, 4>, 64);
     else if (Me <= 96) go(k_tau_solve_wave<3, 32, 8>, 32);
     else if (Me <= 128) go(k_tau_solve_wave<2, 64, 4>, 64);

@@ -183,9 +183,10 @@ struct PressArgs {
     TauSet set;
     double nu;
 };
-void launch_pressure(const Geom& g, const YMats& ym, const PressArgs& pa, hipStream_t st);
+void launch_pressure(const Geom& g, const YMats& ym, const double* hb, const PressArgs& pa, hipStream_t st);
 
-void launch_tau_solve(const Geom& g, int n, const TauMember* tab, hipStream_t st);
+// hb: rows of the Helmholtz systems' operator B, [3][Ny] (Geometry::HB)
+void launch_tau_solve(const Geom& g, const double* hb, int n, const TauMember* tab, hipStream_t st);
 
 // ---- elementwise ----------------------------------------------------------
 // out = sum_j coef[j] * in[j], over n doubles (up to 8 terms)

@@ -221,7 +221,7 @@ int cfg_pressure(const cfg_grid* grid, const cfg_flags* flags, const double* Uba
         ex.run({&ops});
         auto tau = tau_for(*ex.geo, fl.nu, 1.0, true, ex.st, ex.pool, nullptr);
         if (tau->ready) HIPCHK(hipStreamWaitEvent(ex.st, tau->ready, 0));
-        launch_pressure(g, ex.geo->ymats(), PressArgs{f.p, u.p, q.p, tau->set, fl.nu}, ex.st);
+        launch_pressure(g, ex.geo->ymats(), ex.geo->HB.p, PressArgs{f.p, u.p, q.p, tau->set, fl.nu}, ex.st);
         HIPCHK(hipGetLastError());
         download_packed(g, 1, q.p, ex.nscal, q_spec, ex.st);
     });

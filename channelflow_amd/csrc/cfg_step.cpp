@@ -575,7 +575,7 @@ void Exec::issue() {
             }
             case LaunchRec::TAU: {
                 hipEvent_t ev = kt.begin();
-                launch_tau_solve(g, r.n, static_cast<const TauMember*>(r.tab), st);
+                launch_tau_solve(g, geo->HB.p, r.n, static_cast<const TauMember*>(r.tab), st);
                 kt.end(K_TAU, ev);
                 break;
             }

@@ -28,23 +28,76 @@ struct WV {
 struct WCtx {
     int N, Me, Mo, Nb, Nc, lane;  // lane: index within the mode's segment
     double L;
+    // rows of the Chebyshev-tau operator B of the Helmholtz systems by
+    // coefficient n: hb[n], hb[N + n], hb[2N + n] multiply f[n-2], f[n], f[n+2]
+    // (zero on rows n < 2 and where B has no entry)
+    const double* hb;
 };
 
+// DPP move of a double within rows of 16 lanes (a VALU operation, no LDS):
+// lanes whose source lies outside the row get 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double v) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+constexpr int DPP_ROW_SHL = 0x100, DPP_ROW_SHR = 0x110;      // + shift 1..15
+constexpr int DPP_QUAD_XOR1 = 0xB1, DPP_QUAD_XOR2 = 0x4E;    // quad_perm [1,0,3,2], [2,3,0,1]
+constexpr int DPP_ROW_MIRROR = 0x140, DPP_ROW_HALF_MIRROR = 0x141;
+
+// v of segment lane + d (wdn) / lane - d (wup). A lane whose source is outside
+// the segment gets its own v (0 in 16-lane segments, which are DPP rows);
+// every caller discards that value.
 template <int W>
-__device__ __forceinline__ double wdn(double v, int d) { return __shfl_down(v, d, W); }
+__device__ __forceinline__ double wdn(double v, int d) {
+    if constexpr (W == 16) {
+        switch (d) {
+            case 1: return dpp_row<DPP_ROW_SHL + 1>(v);
+            case 2: return dpp_row<DPP_ROW_SHL + 2>(v);
+            case 4: return dpp_row<DPP_ROW_SHL + 4>(v);
+            case 8: return dpp_row<DPP_ROW_SHL + 8>(v);
+            default: break;
+        }
+    }
+    return __shfl_down(v, d, W);
+}
 template <int W>
-__device__ __forceinline__ double wup(double v, int d) { return __shfl_up(v, d, W); }
+__device__ __forceinline__ double wup(double v, int d) {
+    if constexpr (W == 16) {
+        switch (d) {
+            case 1: return dpp_row<DPP_ROW_SHR + 1>(v);
+            case 2: return dpp_row<DPP_ROW_SHR + 2>(v);
+            case 4: return dpp_row<DPP_ROW_SHR + 4>(v);
+            case 8: return dpp_row<DPP_ROW_SHR + 8>(v);
+            default: break;
+        }
+    }
+    return __shfl_up(v, d, W);
+}
 template <int W>
 __device__ __forceinline__ double wbc(double v, int src) { return __shfl(v, src, W); }
 
-// sum over the segment, result identical on every lane (tree to lane 0, broadcast)
+// sum over the segment, result identical on every lane. W = 16: butterfly over
+// DPP partners (quad xor 1, xor 2, half-row mirror, row mirror); each level
+// adds a lane's value to its partner's, and a + b = b + a bit for bit, so all
+// lanes end with the same sum. Otherwise a tree to lane 0 and a broadcast.
 template <int W>
 __device__ __forceinline__ double wsum(double v, int lane) {
-    for (int d = W / 2; d >= 1; d >>= 1) {
-        const double y = wdn<W>(v, d);
-        if (lane + d < W) v += y;
+    if constexpr (W == 16) {
+        v += dpp_row<DPP_QUAD_XOR1>(v);
+        v += dpp_row<DPP_QUAD_XOR2>(v);
+        v += dpp_row<DPP_ROW_HALF_MIRROR>(v);
+        v += dpp_row<DPP_ROW_MIRROR>(v);
+        return v;
+    } else {
+        for (int d = W / 2; d >= 1; d >>= 1) {
+            const double y = wdn<W>(v, d);
+            if (lane + d < W) v += y;
+        }
+        return wbc<W>(v, 0);
     }
-    return wbc<W>(v, 0);
 }
 
 template <int K, int W>
@@ -225,131 +278,6 @@ __device__ __forceinline__ void waff_prefix(double (&cc)[K], const double (&aa)[
     for (int k = 0; k < K; ++k) cc[k] = Gc[k] + Ga[k] * ec;
 }
 
-__device__ __forceinline__ int wh_c(int n, int Nc) { return (n == 0 || n == Nc) ? 2 : 1; }   // helmholtz.h:66
-__device__ __forceinline__ int wh_beta(int n, int Nc) { return (n > Nc - 2) ? 0 : 1; }      // helmholtz.h:65
-
-// One parity system of nu u'' - lambda u = f with u[0] set to bc:
-// x = A^{-1} (B f) using the UL factors band/up/invd of this parity
-// (helmholtz.cpp:79-95 with bandedtridiag.cpp:232-277).
-template <int K, int W>
-__device__ __forceinline__ void whelm_par(const double (&f)[K], const double (&band)[K], const double (&up)[K],
-                                          const double (&invd)[K], double lambda, double bc, int par, int M,
-                                          const WCtx& c, double (&x)[K]) {
-    const int lane = c.lane, Nc = c.Nc;
-    // neighbours f[i-1], f[i+1]
-    double fm[K], fp[K];
-    for (int k = 0; k < K; ++k) {
-        fm[k] = (k > 0) ? f[k - 1] : 0.0;
-        fp[k] = (k + 1 < K) ? f[k + 1] : 0.0;
-    }
-    {
-        const double lastp = wup<W>(f[K - 1], 1);
-        const double firstn = wdn<W>(f[0], 1);
-        if (lane > 0) fm[0] = lastp;
-        if (lane < W - 1) fp[K - 1] = firstn;
-        else fp[K - 1] = 0.0;
-    }
-    // u = B f (rows 1..M-1), u[0] = bc
-    double cc[K], aa[K];
-    for (int k = 0; k < K; ++k) {
-        const int i = K * lane + k;
-        const int n = 2 * i + par;
-        double u = 0.0;
-        if (i == 0) {
-            u = bc;
-        } else if (i < M) {
-            const int nn = n * n;
-            const double lo = ((double)wh_c(n - 2, Nc)) / (4 * n * (n - 1));
-            const double dg = -((double)wh_beta(n, Nc)) / (2 * (nn - 1));
-            if (i < M - 1) {
-                const double upv = wh_beta(n + 2, Nc) ? 1.0 / (4 * n * (n + 1)) : 0.0;
-                u = lo * fm[k] + dg * f[k] + upv * fp[k];
-            } else {
-                u = lo * fm[k] + dg * f[k];
-            }
-        }
-        x[k] = u;
-        // backward substitution maps on rows 1..M-1
-        if (i >= 1 && i < M - 1) {
-            cc[k] = u;
-            aa[k] = -up[k];
-        } else if (i == M - 1 && i >= 1) {
-            cc[k] = u;
-            aa[k] = 0.0;
-        } else {
-            cc[k] = 0.0;
-            aa[k] = 1.0;
-        }
-    }
-    waff_suffix<K, W>(cc, aa, lane);
-    // row 0: x0 = (u0 - sum_{j>=1} band_j x_j) / band_0
-    double part = 0.0;
-    for (int k = 0; k < K; ++k) {
-        const int i = K * lane + k;
-        if (i >= 1 && i < M) part += band[k] * cc[k];
-    }
-    const double s = wsum<W>(part, lane);
-    const double u0 = wbc<W>(x[0], 0), b0 = wbc<W>(band[0], 0);
-    const double x0 = (u0 - s) / b0;
-    // forward substitution y_i = (x_i - lo_i y_{i-1}) invd_i
-    for (int k = 0; k < K; ++k) {
-        const int i = K * lane + k;
-        const int n = 2 * i + par;
-        if (i == 0) {
-            cc[k] = x0;
-            aa[k] = 0.0;
-        } else if (i < M) {
-            const double lo = -(wh_c(n - 2, Nc) * lambda) / (4 * n * (n - 1));
-            cc[k] = cc[k] * invd[k];
-            aa[k] = -lo * invd[k];
-        } else {
-            cc[k] = 0.0;
-            aa[k] = 1.0;
-        }
-    }
-    waff_prefix<K, W>(cc, aa, lane);
-    for (int k = 0; k < K; ++k) x[k] = (K * lane + k < M) ? cc[k] : 0.0;
-}
-
-// Helmholtz solve of a full profile with Dirichlet values ua (y=a), ub (y=b);
-// fac points at the packed factors band[N], up[N], invd[N] (contiguous).
-template <int K, int W>
-__device__ __forceinline__ WV<K> whelm(const WV<K>& f, const double* band, const double* up, const double* invd,
-                                       double lambda, double ua, double ub, const WCtx& c) {
-    const WV<K> B = wload<K, W>(band, c), U = wload<K, W>(up, c), I = wload<K, W>(invd, c);
-    WV<K> x;
-    whelm_par<K, W>(f.e, B.e, U.e, I.e, lambda, (ub + ua) / 2, 0, c.Me, c, x.e);
-    whelm_par<K, W>(f.o, B.o, U.o, I.o, lambda, (ub - ua) / 2, 1, c.Mo, c, x.o);
-    return x;
-}
-
-// mean value over [a,b] (chebyshev.cpp:505-512): c0 - sum_{n even >= 2} c_n/(n^2-1)
-template <int K, int W>
-__device__ __forceinline__ double wmean(const WV<K>& v, const WCtx& c) {
-    double part = 0.0;
-    for (int k = 0; k < K; ++k) {
-        const int i = K * c.lane + k;
-        if (i >= 1 && i < c.Me) part += v.e[k] / (double)(4 * i * i - 1);
-    }
-    return wbc<W>(v.e[0], 0) - wsum<W>(part, c.lane);
-}
-
-// Helmholtz solve with unknown constant forcing mu and mean(u) = umean
-// (helmholtz.cpp:158-213); returns mu.
-template <int K, int W>
-__device__ __forceinline__ double whelm_mean(const WV<K>& f, const double* band, const double* up, const double* invd,
-                                             double lambda, double nu, double umean, const WCtx& c, WV<K>& u) {
-    const double uamean = wmean<K, W>(whelm<K, W>(f, band, up, invd, lambda, 0.0, 0.0, c), c);
-    WV<K> r = wzero<K, W>();
-    if (c.lane == 0) r.e[0] = nu;
-    const double ucmean = wmean<K, W>(whelm<K, W>(r, band, up, invd, lambda, 0.0, 0.0, c), c);
-    const double mu = nu * (umean - uamean) / ucmean;
-    r = f;
-    if (c.lane == 0) r.e[0] += mu;
-    u = whelm<K, W>(r, band, up, invd, lambda, 0.0, 0.0, c);
-    return mu;
-}
-
 // ---- several right-hand sides per scan ----------------------------------------
 // The Chebyshev-tau operators are real, so the real and imaginary parts of a
 // mode (and u and w, which share the velocity operator) are solved together:
@@ -434,6 +362,11 @@ __device__ __forceinline__ void waff_prefix_n(double (&cc)[NR][K], const double 
 // sums over the wave of NR values, identical on every lane (as wsum)
 template <int W, int NR>
 __device__ __forceinline__ void wsum_n(double (&v)[NR], int lane) {
+    if constexpr (W == 16) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) v[r] = wsum<W>(v[r], lane);
+        return;
+    }
 #pragma unroll
     for (int d = W / 2; d >= 1; d >>= 1) {
         double y[NR];
@@ -448,12 +381,18 @@ __device__ __forceinline__ void wsum_n(double (&v)[NR], int lane) {
     for (int r = 0; r < NR; ++r) v[r] = wbc<W>(v[r], 0);
 }
 
-// whelm_par for NR right-hand sides f[r] with boundary values bc[r]
+// One parity system of nu u'' - lambda u = f for NR right-hand sides f[r]
+// with u[0] set to bc[r]: x = A^{-1} (B f), B from the rows bl, bd, bu of this
+// parity and A from its UL factors band / up / invd (helmholtz.cpp:79-95 with
+// bandedtridiag.cpp:232-277). One affine scan carries all NR constant terms
+// with a shared multiplier, so each right-hand side gets the arithmetic it
+// would get alone.
 template <int K, int W, int NR>
 __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const double (&band)[K], const double (&up)[K],
-                                            const double (&invd)[K], double lambda, const double (&bc)[NR], int par,
-                                            int M, const WCtx& c, double (&x)[NR][K]) {
-    const int lane = c.lane, Nc = c.Nc;
+                                            const double (&invd)[K], const double (&bl)[K], const double (&bd)[K],
+                                            const double (&bu)[K], double lambda, const double (&bc)[NR], int M,
+                                            const WCtx& c, double (&x)[NR][K]) {
+    const int lane = c.lane;
     double cc[NR][K], aa[K];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -467,26 +406,16 @@ __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const doub
         const double firstn = wdn<W>(f[r][0], 1);
         if (lane > 0) fm[0] = lastp;
         fp[K - 1] = (lane < W - 1) ? firstn : 0.0;
+        // u = B f: zero past the system (zero rows), bc on row 0
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const int i = K * lane + k;
-            const int n = 2 * i + par;
-            double u = 0.0;
-            if (i == 0) {
-                u = bc[r];
-            } else if (i < M) {
-                const int nn = n * n;
-                const double lo = ((double)wh_c(n - 2, Nc)) / (4 * n * (n - 1));
-                const double dg = -((double)wh_beta(n, Nc)) / (2 * (nn - 1));
-                if (i < M - 1) {
-                    const double upv = wh_beta(n + 2, Nc) ? 1.0 / (4 * n * (n + 1)) : 0.0;
-                    u = lo * fm[k] + dg * f[r][k] + upv * fp[k];
-                } else {
-                    u = lo * fm[k] + dg * f[r][k];
-                }
-            }
+            const double u = bl[k] * fm[k] + bd[k] * f[r][k] + bu[k] * fp[k];
             x[r][k] = u;
-            cc[r][k] = (i >= 1 && i < M) ? u : 0.0;
+            cc[r][k] = u;
+        }
+        if (lane == 0) {
+            x[r][0] = bc[r];
+            cc[r][0] = 0.0;  // row 0 is not a backward-substitution map
         }
     }
 #pragma unroll
@@ -495,41 +424,35 @@ __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const doub
         aa[k] = (i >= 1 && i < M - 1) ? -up[k] : ((i == M - 1 && i >= 1) ? 0.0 : 1.0);
     }
     waff_suffix_n<K, W, NR>(cc, aa, lane);
-    // row 0: x0 = (u0 - sum_{j>=1} band_j x_j) / band_0
+    // row 0: x0 = (u0 - sum_{j>=1} band_j x_j) / band_0 (band is zero past the system)
     double part[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         part[r] = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const int i = K * lane + k;
-            if (i >= 1 && i < M) part[r] += band[k] * cc[r][k];
+            if (k == 0 && lane == 0) continue;
+            part[r] += band[k] * cc[r][k];
         }
     }
     wsum_n<W, NR>(part, lane);
-    const double b0 = wbc<W>(band[0], 0);
+    // x0 is used by lane 0 only (row i = 0), which holds u0 = x[r][0] and band_0
     double x0[NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) x0[r] = (wbc<W>(x[r][0], 0) - part[r]) / b0;
-    // forward substitution y_i = (x_i - lo_i y_{i-1}) invd_i
+    for (int r = 0; r < NR; ++r) x0[r] = (x[r][0] - part[r]) / band[0];
+    // forward substitution y_i = (x_i - lo_i y_{i-1}) invd_i with lo_i = -lambda bl_i
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int i = K * lane + k;
-        const int n = 2 * i + par;
-        if (i == 0) {
-            aa[k] = 0.0;
+        const bool in = i < M;
+        aa[k] = in ? (lambda * bl[k]) * invd[k] : 1.0;
 #pragma unroll
-            for (int r = 0; r < NR; ++r) cc[r][k] = x0[r];
-        } else if (i < M) {
-            const double lo = -(wh_c(n - 2, Nc) * lambda) / (4 * n * (n - 1));
-            aa[k] = -lo * invd[k];
+        for (int r = 0; r < NR; ++r) cc[r][k] = in ? cc[r][k] * invd[k] : 0.0;
+    }
+    if (lane == 0) {
+        aa[0] = 0.0;
 #pragma unroll
-            for (int r = 0; r < NR; ++r) cc[r][k] = cc[r][k] * invd[k];
-        } else {
-            aa[k] = 1.0;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) cc[r][k] = 0.0;
-        }
+        for (int r = 0; r < NR; ++r) cc[r][0] = x0[r];
     }
     waff_prefix_n<K, W, NR>(cc, aa, lane);
 #pragma unroll
@@ -538,12 +461,14 @@ __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const doub
         for (int k = 0; k < K; ++k) x[r][k] = (K * lane + k < M) ? cc[r][k] : 0.0;
 }
 
-// whelm for NR profiles sharing one operator; Dirichlet values ua[r], ub[r]
+// Helmholtz solve of NR profiles sharing one operator with Dirichlet values
+// ua[r] (y=a), ub[r] (y=b); band, up, invd: the operator's UL factors, N each.
 template <int K, int W, int NR>
 __device__ __forceinline__ void whelm_n(const WV<K> (&f)[NR], const double* band, const double* up, const double* invd,
                                         double lambda, const double (&ua)[NR], const double (&ub)[NR], const WCtx& c,
                                         WV<K> (&x)[NR]) {
     const WV<K> B = wload<K, W>(band, c), U = wload<K, W>(up, c), I = wload<K, W>(invd, c);
+    const WV<K> HL = wload<K, W>(c.hb, c), HD = wload<K, W>(c.hb + c.N, c), HU = wload<K, W>(c.hb + 2 * c.N, c);
     double fe[NR][K], fo[NR][K], xe[NR][K], xo[NR][K], be[NR], bo[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -555,8 +480,8 @@ __device__ __forceinline__ void whelm_n(const WV<K> (&f)[NR], const double* band
             fo[r][k] = f[r].o[k];
         }
     }
-    whelm_par_n<K, W, NR>(fe, B.e, U.e, I.e, lambda, be, 0, c.Me, c, xe);
-    whelm_par_n<K, W, NR>(fo, B.o, U.o, I.o, lambda, bo, 1, c.Mo, c, xo);
+    whelm_par_n<K, W, NR>(fe, B.e, U.e, I.e, HL.e, HD.e, HU.e, lambda, be, c.Me, c, xe);
+    whelm_par_n<K, W, NR>(fo, B.o, U.o, I.o, HL.o, HD.o, HU.o, lambda, bo, c.Mo, c, xo);
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
@@ -565,6 +490,45 @@ __device__ __forceinline__ void whelm_n(const WV<K> (&f)[NR], const double* band
             x[r].o[k] = xo[r][k];
         }
 }
+
+// one profile
+template <int K, int W>
+__device__ __forceinline__ WV<K> whelm(const WV<K>& f, const double* band, const double* up, const double* invd,
+                                       double lambda, double ua, double ub, const WCtx& c) {
+    const WV<K> f1[1] = {f};
+    const double a1[1] = {ua}, b1[1] = {ub};
+    WV<K> x1[1];
+    whelm_n<K, W, 1>(f1, band, up, invd, lambda, a1, b1, c, x1);
+    return x1[0];
+}
+
+// mean value over [a,b] (chebyshev.cpp:505-512): c0 - sum_{n even >= 2} c_n/(n^2-1)
+template <int K, int W>
+__device__ __forceinline__ double wmean(const WV<K>& v, const WCtx& c) {
+    double part = 0.0;
+    for (int k = 0; k < K; ++k) {
+        const int i = K * c.lane + k;
+        if (i >= 1 && i < c.Me) part += v.e[k] / (double)(4 * i * i - 1);
+    }
+    return wbc<W>(v.e[0], 0) - wsum<W>(part, c.lane);
+}
+
+// Helmholtz solve with unknown constant forcing mu and mean(u) = umean
+// (helmholtz.cpp:158-213); returns mu.
+template <int K, int W>
+__device__ __forceinline__ double whelm_mean(const WV<K>& f, const double* band, const double* up, const double* invd,
+                                             double lambda, double nu, double umean, const WCtx& c, WV<K>& u) {
+    const double uamean = wmean<K, W>(whelm<K, W>(f, band, up, invd, lambda, 0.0, 0.0, c), c);
+    WV<K> r = wzero<K, W>();
+    if (c.lane == 0) r.e[0] = nu;
+    const double ucmean = wmean<K, W>(whelm<K, W>(r, band, up, invd, lambda, 0.0, 0.0, c), c);
+    const double mu = nu * (umean - uamean) / ucmean;
+    r = f;
+    if (c.lane == 0) r.e[0] += mu;
+    u = whelm<K, W>(r, band, up, invd, lambda, 0.0, 0.0, c);
+    return mu;
+}
+
 
 // suffix sums of NR arrays at once (as wsuffix)
 template <int K, int W, int NR>

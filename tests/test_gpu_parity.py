@@ -65,8 +65,11 @@ def test_nonlinear_single_call(gpu, method, dealias, grid):
 
 @pytest.mark.parametrize("constraint", ["PressureGradient", "BulkVelocity"])
 @pytest.mark.parametrize("taucorr", [True, False])
-def test_solve_single_call(gpu, constraint, taucorr):
-    Nx, Ny, Nz = 16, 33, 16
+@pytest.mark.parametrize("grid", [(16, 33, 16), (8, 25, 8), (8, 65, 8), (8, 97, 8), (8, 129, 8), (6, 193, 6)])
+def test_solve_single_call(gpu, constraint, taucorr, grid):
+    """Ny = 25, 33, 65 take 16-lane segments with 1, 2, 3 even coefficients per
+    lane; 97, 129, 193 the 64- and 32-lane segments."""
+    Nx, Ny, Nz = grid
     g = R.Grid(Nx, Ny, Nz, LX_C, LZ_C)
     rng = np.random.default_rng(5)
     rhs = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=1.0, decay=0.7, seed=3) + \
