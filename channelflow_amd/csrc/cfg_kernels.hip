@@ -6,11 +6,12 @@
 //   k_yinv_t Chebyshev coefficients -> values at the Chebyshev points y_n,
 //            with d/dy folded into the operator (T, T*D) and the base flow
 //            U(y), W(y), -Vsuck added to the kx=kz=0 mode;
-//   k_xinv   inverse x-FFT per (field, y, kz) line, optional d/dx;
+//   k_xwinv  inverse x-FFT per (field, y, kz) line, optional d/dx (k_xinv:
+//            LDS FFTs for the lengths the wavefront FFT does not cover);
 //   k_zwave  per (y, x) line: c2r z-FFTs, pointwise products, r2c z-FFTs,
 //            a wavefront per group of lines (k_zpass: LDS FFTs, block-wide,
 //            for the lengths the wavefront FFT does not cover);
-//   k_xfwd   forward x-FFT, live kx kept, 1/(Nx Nz) scaling;
+//   k_xwfwd  forward x-FFT, live kx kept, 1/(Nx Nz) scaling (k_xfwd);
 //   k_yfwd_t values -> Chebyshev coefficients, with the spectral divergence
 //            of u_i u_j folded in for the divergence/skew-symmetric forms.
 // Only non-aliased modes |kx| <= kxd, kz <= kzd are ever stored, so the
@@ -1279,6 +1280,59 @@ struct ZWCtx {
     const double2* tw; // exp(-2 pi i m / N), m < N, in LDS
 };
 
+// DFT (sign SIGN) of a line held in layout A, v[r] = Z[j + N1 r] on lane
+// j < N1, into layout B, u[m] = X[j + N2 m] on lane j < N2:
+// X[k1 + N2 k2] = sum_j w_N1^(j k2) w_N^(j k1) sum_r Z[j + N1 r] w_N2^(r k1).
+template <int N, int SIGN>
+__device__ __forceinline__ void zw_a2b(double2 (&v)[ZW<N>::N2], double2 (&u)[ZW<N>::N1], int j, double2* T,
+                                       const double2* tw) {
+    constexpr int N1 = ZW<N>::N1, N2 = ZW<N>::N2;
+    dft_reg<N2, SIGN>(v);
+    const int jt = j < N1 ? j : N1 - 1;
+#pragma unroll
+    for (int k1 = 1; k1 < N2; ++k1) {
+        double2 t = tw[jt * k1];
+        if (SIGN > 0) t.y = -t.y;
+        v[k1] = cmul(v[k1], t);
+    }
+    wave_lds_sync();
+    if (j < N1) {
+#pragma unroll
+        for (int k1 = 0; k1 < N2; ++k1) T[k1 * (N1 + 1) + j] = v[k1];
+    }
+    wave_lds_sync();
+    const int row = j < N2 ? j : N2 - 1;
+#pragma unroll
+    for (int i = 0; i < N1; ++i) u[i] = T[row * (N1 + 1) + i];
+    dft_reg<N1, SIGN>(u);
+}
+
+// the same from layout B, u[m] = x[j + N2 m] on lane j < N2, into layout A,
+// v[p] = X[j + N1 p] on lane j < N1
+template <int N, int SIGN>
+__device__ __forceinline__ void zw_b2a(double2 (&u)[ZW<N>::N1], double2 (&v)[ZW<N>::N2], int j, double2* T,
+                                       const double2* tw) {
+    constexpr int N1 = ZW<N>::N1, N2 = ZW<N>::N2;
+    dft_reg<N1, SIGN>(u);
+    const int jt = j < N2 ? j : N2 - 1;
+#pragma unroll
+    for (int q = 1; q < N1; ++q) {
+        double2 t = tw[jt * q];
+        if (SIGN > 0) t.y = -t.y;
+        u[q] = cmul(u[q], t);
+    }
+    wave_lds_sync();
+    if (j < N2) {
+#pragma unroll
+        for (int q = 0; q < N1; ++q) T[q * (N2 + 1) + j] = u[q];
+    }
+    wave_lds_sync();
+    const int row = j < N1 ? j : N1 - 1;
+#pragma unroll
+    for (int i = 0; i < N2; ++i) v[i] = T[row * (N2 + 1) + i];
+    dft_reg<N2, SIGN>(v);
+}
+
 // c2r of two real fields at once (fa -> pa, fb -> pb; fb < 0: none), from the
 // live bins kz < nkz of xp; dza / dzb apply d/dz. As in FFTW's c2r, the
 // imaginary part of bin 0 is ignored.
@@ -1305,25 +1359,8 @@ __device__ __forceinline__ void zw_c2r(const ZWCtx& c, const double2* __restrict
         else if (lo) v[r] = make_double2(A.x - B.y, A.y + B.x);  // A + iB
         else v[r] = make_double2(A.x + B.y, B.x - A.y);          // conj(A) + i conj(B) (zero in the gap)
     }
-    dft_reg<N2, +1>(v);
-    const int jt = j < N1 ? j : N1 - 1;
-#pragma unroll
-    for (int k1 = 1; k1 < N2; ++k1) {
-        double2 t = c.tw[jt * k1];
-        t.y = -t.y;
-        v[k1] = cmul(v[k1], t);
-    }
-    wave_lds_sync();
-    if (j < N1) {
-#pragma unroll
-        for (int k1 = 0; k1 < N2; ++k1) c.T[k1 * (N1 + 1) + j] = v[k1];
-    }
-    wave_lds_sync();
-    const int row = j < N2 ? j : N2 - 1;
     double2 u[N1];
-#pragma unroll
-    for (int i = 0; i < N1; ++i) u[i] = c.T[row * (N1 + 1) + i];
-    dft_reg<N1, +1>(u);
+    zw_a2b<NZ, +1>(v, u, j, c.T, c.tw);
 #pragma unroll
     for (int m = 0; m < N1; ++m) {
         pa[m] = u[m].x;
@@ -1340,21 +1377,8 @@ __device__ __forceinline__ void zw_r2c(const ZWCtx& c, double2* __restrict__ zo,
     double2 v[N1];
 #pragma unroll
     for (int m = 0; m < N1; ++m) v[m] = make_double2(pa[m], TWO ? pb[m] : 0.0);
-    dft_reg<N1, -1>(v);
-    const int jt = j < N2 ? j : N2 - 1;
-#pragma unroll
-    for (int q = 1; q < N1; ++q) v[q] = cmul(v[q], c.tw[jt * q]);
-    wave_lds_sync();
-    if (j < N2) {
-#pragma unroll
-        for (int q = 0; q < N1; ++q) c.T[q * (N2 + 1) + j] = v[q];
-    }
-    wave_lds_sync();
-    const int row = j < N1 ? j : N1 - 1;
     double2 w[N2];
-#pragma unroll
-    for (int i = 0; i < N2; ++i) w[i] = c.T[row * (N2 + 1) + i];
-    dft_reg<N2, -1>(w);  // w[p] = X[row + N1 p]
+    zw_b2a<NZ, -1>(v, w, j, c.T, c.tw);  // w[p] = X[j + N1 p]
     wave_lds_sync();
     if (j < N1) {
 #pragma unroll
@@ -1524,6 +1548,101 @@ __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict
             pb[m] = u2[m] * u2[m];
         }
         zw_r2c<NZ, true>(c, zo, pa, pb, 7, 8);
+    }
+}
+
+// ---- wave-local x FFTs ---------------------------------------------------------
+// One wavefront per 64/G consecutive kz-lines of one (field, y); each line's
+// FFT as in the wave-local z pass. Consecutive lanes of a group hold
+// consecutive kx (contiguous reads of yp / writes of fo); the lines of a
+// wavefront are consecutive kz (128-byte runs of xp / zo).
+
+// inverse x-FFT: yp [F][Ny][nkz][nkx] -> xp [Fo][Ny][Nx][nkz]; output f >= nsrc
+// is d/dx of source field f - nsrc
+template <int NX>
+__global__ void __launch_bounds__(256) k_xwinv(Geom g, const double2* __restrict__ twg, int nsrc,
+                                               const double2* __restrict__ yp0, long sy, double2* __restrict__ xp0,
+                                               long sx) {
+    using S = ZW<NX>;
+    constexpr int N1 = S::N1, N2 = S::N2;
+    extern __shared__ double2 lds[];
+    for (int i = threadIdx.x; i < NX; i += blockDim.x) lds[i] = twg[i];
+    __syncthreads();
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
+    const int nzc = (nkz + S::LPW - 1) / S::LPW;
+    const int wg = blockIdx.x * 4 + wave;
+    const int y = wg / nzc;
+    if (y >= Ny) return;
+    const int L = lane / S::G, j = lane % S::G;
+    const int ikz = (wg - y * nzc) * S::LPW + L;
+    const bool line_ok = ikz < nkz;
+    const int fo = blockIdx.y;
+    const bool deriv = fo >= nsrc;
+    const int fs = deriv ? fo - nsrc : fo;
+    const double2* __restrict__ src = yp0 + blockIdx.z * sy + (((long)fs * Ny + y) * nkz + (line_ok ? ikz : 0)) * nkx;
+    const double dk = 2 * PI / g.Lx;
+    double2 v[N2];
+#pragma unroll
+    for (int r = 0; r < N2; ++r) {
+        const int n = j + N1 * r;
+        int ikx = -1;
+        if (n <= g.kxd) ikx = n;
+        else if (n >= NX - g.kxd) ikx = n - NX + nkx;
+        double2 z = make_double2(0.0, 0.0);
+        if (line_ok && j < N1 && ikx >= 0) {
+            z = src[ikx];
+            if (deriv) z = imul(dk * kx_of(g, ikx), z);
+        }
+        v[r] = z;
+    }
+    double2 u[N1];
+    zw_a2b<NX, +1>(v, u, j, lds + NX + (wave * S::LPW + L) * S::LBUF, lds);
+    if (line_ok && j < N2) {
+        double2* __restrict__ out = xp0 + blockIdx.z * sx + ((long)fo * Ny + y) * NX * (long)nkz + ikz;
+#pragma unroll
+        for (int m = 0; m < N1; ++m) out[(long)(j + N2 * m) * nkz] = u[m];
+    }
+}
+
+// forward x-FFT: zo [G][Ny][Nx][nkz] -> fo [G][Ny][nkz][nkx], scaled by 1/(Nx Nz)
+template <int NX>
+__global__ void __launch_bounds__(256) k_xwfwd(Geom g, const double2* __restrict__ twg,
+                                               const double2* __restrict__ zo0, long sx, double2* __restrict__ fo0,
+                                               long sy) {
+    using S = ZW<NX>;
+    constexpr int N1 = S::N1, N2 = S::N2;
+    extern __shared__ double2 lds[];
+    for (int i = threadIdx.x; i < NX; i += blockDim.x) lds[i] = twg[i];
+    __syncthreads();
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
+    const int nzc = (nkz + S::LPW - 1) / S::LPW;
+    const int wg = blockIdx.x * 4 + wave;
+    const int y = wg / nzc;
+    if (y >= Ny) return;
+    const int L = lane / S::G, j = lane % S::G;
+    const int ikz = (wg - y * nzc) * S::LPW + L;
+    const bool line_ok = ikz < nkz;
+    const int f = blockIdx.y;
+    const double2* __restrict__ in =
+        zo0 + blockIdx.z * sx + ((long)f * Ny + y) * NX * (long)nkz + (line_ok ? ikz : 0);
+    double2 u[N1];
+#pragma unroll
+    for (int m = 0; m < N1; ++m) u[m] = (line_ok && j < N2) ? in[(long)(j + N2 * m) * nkz] : make_double2(0.0, 0.0);
+    double2 v[N2];
+    zw_b2a<NX, -1>(u, v, j, lds + NX + (wave * S::LPW + L) * S::LBUF, lds);
+    if (line_ok && j < N1) {
+        const double scale = 1.0 / ((double)g.Nx * g.Nz);
+        double2* __restrict__ out = fo0 + blockIdx.z * sy + (((long)f * Ny + y) * nkz + ikz) * nkx;
+#pragma unroll
+        for (int p = 0; p < N2; ++p) {
+            const int n = j + N1 * p;
+            int ikx = -1;
+            if (n <= g.kxd) ikx = n;
+            else if (n >= NX - g.kxd) ikx = n - NX + nkx;
+            if (ikx >= 0) out[ikx] = make_double2(v[p].x * scale, v[p].y * scale);
+        }
     }
 }
 
@@ -2181,6 +2300,20 @@ static void allow_lds(K kernel, size_t bytes) {
 template <int NX>
 static void launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, int n, const double2* in,
                      long sin, double2* out, long sout, const NLTune& tn, hipStream_t st) {
+    if constexpr (zwave_size(NX)) {
+        using S = ZW<NX>;
+        const long waves = (long)g.Ny * ((g.nkz + S::LPW - 1) / S::LPW);
+        const size_t lds = (size_t)(NX + 4 * S::LPW * S::LBUF) * sizeof(double2);
+        const dim3 grid((unsigned)((waves + 3) / 4), nf, n);
+        if (inverse) {
+            allow_lds(k_xwinv<NX>, lds);
+            k_xwinv<NX><<<grid, 256, lds, st>>>(g, px.tw, nsrc, in, sin, out, sout);
+        } else {
+            allow_lds(k_xwfwd<NX>, lds);
+            k_xwfwd<NX><<<grid, 256, lds, st>>>(g, px.tw, in, sin, out, sout);
+        }
+        return;
+    }
     const int XL = xfft_lines(g, tn);
     const int nchunk = (g.nkz + XL - 1) / XL;
     const size_t lds = (size_t)(g.Nx + 2 * XL * lpitch(g.Nx)) * sizeof(double2);
@@ -2387,7 +2520,8 @@ NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
         }
     }
     float bx = 1e30f;
-    for (int xl = 1; xl <= xfft_lines_max(g); xl *= 2) {
+    const int xl_max = zwave_size(g.Nx) ? 0 : xfft_lines_max(g);  // the wavefront x-FFTs have no tiling
+    for (int xl = 1; xl <= xl_max; xl *= 2) {
         NLTune c;
         c.xl = xl;
         const float ms = timed([&] { launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, n, yp, sy,

@@ -42,11 +42,15 @@ def live_mask(Nx, Nz, kxd, kzd):
 @pytest.mark.parametrize("method", ["Rotational", "Convection", "Divergence", "SkewSymmetric"])
 @pytest.mark.parametrize("dealias", ["DealiasXZ", "NoDealiasing"])
 @pytest.mark.parametrize("grid", [(24, 17, 18), (16, 17, 24), (32, 17, 48), (16, 19, 192), (8, 9, 16), (12, 11, 32),
-                                  (12, 13, 64), (8, 9, 96), (8, 9, 128), (4, 7, 256)])
+                                  (12, 13, 64), (8, 9, 96), (8, 9, 128), (4, 7, 256), (48, 9, 16), (64, 9, 12),
+                                  (96, 9, 8), (128, 7, 16), (192, 7, 8), (256, 7, 8)])
 def test_nonlinear_single_call(gpu, method, dealias, grid):
-    """Nz = 18 takes the runtime-length LDS z-pass, 24 the compile-time LDS one,
-    the others the wavefront z-pass (Nx = 12 at Nz = 32 and 64, and Nx = 8 at
-    Nz = 16, leave the last wavefront of each y partly without lines)."""
+    """z-pass: Nz = 18 takes the runtime-length LDS FFTs, 24 the compile-time
+    LDS ones, the others the wavefront FFTs (Nx = 12 at Nz = 32 and 64, and
+    Nx = 8 at Nz = 16, leave the last wavefront of each y partly without
+    lines). x-FFTs: Nx = 16, 32, 48, 64, 96, 128, 192, 256 take the wavefront
+    FFTs (lines are kz, so most of these end in a partial wavefront), the
+    others the LDS ones."""
     Nx, Ny, Nz = grid
     g = R.Grid(Nx, Ny, Nz, LX_C, LZ_C)
     u = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.3, decay=0.5, seed=11, dealias=(dealias == "DealiasXZ"))
