@@ -119,8 +119,9 @@ def kernel_bytes(grid, method):
         "k_zpass": nxout * X + nzout * X,
         "k_xfwd": nzout * X + nzout * L,
         "k_yfwd": nzout * L + 3 * L,
-        # SBDF3 right-hand side u_n..u_{n-2}, f_n..f_{n-2} read; u (3) and q (1) written
-        "k_tau_solve": (18 + 4) * L,
+        # SBDF3 right-hand side u_n..u_{n-2}, f_n..f_{n-2} read; the per-mode tau
+        # factors (12 Ny-vectors of doubles per mode = 6 L) read; u (3) and q (1) written
+        "k_tau_solve": (18 + 6 + 4) * L,
     }
 
 
@@ -462,6 +463,10 @@ def copy_calibration(nbytes=COPY_CAL_BYTES, reps=3):
     print(f"copy calibration: {reps} device copies of {nbytes} bytes", file=sys.stderr, flush=True)
 
 
+# device kernels that fill a step slot of another name (wavefront-local FFT passes)
+KERNEL_ALIAS = {"k_zwave": "k_zpass", "k_xwinv": "k_xinv", "k_xwfwd": "k_xfwd"}
+
+
 def _counter_rows(path, counter):
     """(short kernel name, value) per dispatch from a rocprofv3 counter_collection.csv."""
     import csv
@@ -473,7 +478,8 @@ def _counter_rows(path, counter):
                 continue
             name = r["Kernel_Name"]
             m = re.search(r"cfg::(k_\w+?)(?:_wave|_w|_t)?(?:<|\()", name)  # variants count as their kernel
-            rows.append((m.group(1) if m else name.split("(")[0].strip(), float(r["Counter_Value"])))
+            short = m.group(1) if m else name.split("(")[0].strip()
+            rows.append((KERNEL_ALIAS.get(short, short), float(r["Counter_Value"])))
     return rows
 
 

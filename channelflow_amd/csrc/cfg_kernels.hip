@@ -1400,7 +1400,7 @@ __device__ __forceinline__ void zw_r2c(const ZWCtx& c, double2* __restrict__ zo,
 }
 
 // The z pass of k_zpass with one wavefront per 64/G lines (same fields, same
-// products in the same order). Block: 4 independent wavefronts.
+// products in the same order). A block is 1, 2 or 4 independent wavefronts.
 template <int METHOD, int NZ>
 __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict__ twg, const NLMember* __restrict__ tab,
                                                int want_g, const double2* __restrict__ xp0, double2* __restrict__ zo0,
@@ -1412,7 +1412,7 @@ __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict
     __syncthreads();
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int nxc = (g.Nx + S::LPW - 1) / S::LPW;
-    const int wg = blockIdx.x * 4 + wave;
+    const int wg = blockIdx.x * (blockDim.x / 64) + wave;
     const int y = wg / nxc;
     if (y >= g.Ny) return;
     const int L = lane / S::G;
@@ -1571,7 +1571,7 @@ __global__ void __launch_bounds__(256) k_xwinv(Geom g, const double2* __restrict
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
     const int nzc = (nkz + S::LPW - 1) / S::LPW;
-    const int wg = blockIdx.x * 4 + wave;
+    const int wg = blockIdx.x * (blockDim.x / 64) + wave;
     const int y = wg / nzc;
     if (y >= Ny) return;
     const int L = lane / S::G, j = lane % S::G;
@@ -1618,7 +1618,7 @@ __global__ void __launch_bounds__(256) k_xwfwd(Geom g, const double2* __restrict
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
     const int nzc = (nkz + S::LPW - 1) / S::LPW;
-    const int wg = blockIdx.x * 4 + wave;
+    const int wg = blockIdx.x * (blockDim.x / 64) + wave;
     const int y = wg / nzc;
     if (y >= Ny) return;
     const int L = lane / S::G, j = lane % S::G;
@@ -2302,15 +2302,16 @@ static void launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, i
                      long sin, double2* out, long sout, const NLTune& tn, hipStream_t st) {
     if constexpr (zwave_size(NX)) {
         using S = ZW<NX>;
+        const int wb = tn.xwb > 0 ? tn.xwb : 4;
         const long waves = (long)g.Ny * ((g.nkz + S::LPW - 1) / S::LPW);
-        const size_t lds = (size_t)(NX + 4 * S::LPW * S::LBUF) * sizeof(double2);
-        const dim3 grid((unsigned)((waves + 3) / 4), nf, n);
+        const size_t lds = (size_t)(NX + wb * S::LPW * S::LBUF) * sizeof(double2);
+        const dim3 grid((unsigned)((waves + wb - 1) / wb), nf, n);
         if (inverse) {
             allow_lds(k_xwinv<NX>, lds);
-            k_xwinv<NX><<<grid, 256, lds, st>>>(g, px.tw, nsrc, in, sin, out, sout);
+            k_xwinv<NX><<<grid, 64 * wb, lds, st>>>(g, px.tw, nsrc, in, sin, out, sout);
         } else {
             allow_lds(k_xwfwd<NX>, lds);
-            k_xwfwd<NX><<<grid, 256, lds, st>>>(g, px.tw, in, sin, out, sout);
+            k_xwfwd<NX><<<grid, 64 * wb, lds, st>>>(g, px.tw, in, sin, out, sout);
         }
         return;
     }
@@ -2344,11 +2345,12 @@ static void launch_z_t(const Geom& g, const FFTPlan& pz, int n, const NLMember* 
                        double2* zo, long sx, const NLTune& tn, hipStream_t st) {
     if constexpr (zwave_size(NZ)) {
         using S = ZW<NZ>;
+        const int wb = tn.zwb > 0 ? tn.zwb : 4;
         const long waves = (long)g.Ny * ((g.Nx + S::LPW - 1) / S::LPW);
-        const size_t lds = (size_t)(NZ + 4 * S::LPW * S::LBUF) * sizeof(double2);
-        const dim3 grid((unsigned)((waves + 3) / 4), n);
+        const size_t lds = (size_t)(NZ + wb * S::LPW * S::LBUF) * sizeof(double2);
+        const dim3 grid((unsigned)((waves + wb - 1) / wb), n);
         allow_lds(k_zwave<METHOD, NZ>, lds);
-        k_zwave<METHOD, NZ><<<grid, 256, lds, st>>>(g, pz.tw, tab, want_g, xp, zo, sx);
+        k_zwave<METHOD, NZ><<<grid, 64 * wb, lds, st>>>(g, pz.tw, tab, want_g, xp, zo, sx);
         return;
     }
     const int nphys = zpass_nphys(METHOD);
@@ -2471,9 +2473,10 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     T.end(K_YFWD, ev);
 }
 
-// Autotuner of the nonlinear-term tilings: each candidate lines-per-block of
-// the z-pass and of the x-FFTs is timed (HIP events, best of 3 after a warm
-// launch) on the caller's work buffers and member table. The tiling only
+// Autotuner of the nonlinear-term tilings: each candidate block shape of the
+// z-pass and of the x-FFTs (wavefronts per block for the wavefront FFTs,
+// lines per block for the LDS ones) is timed (HIP events, best of 3 after a
+// warm launch) on the caller's work buffers and member table. The tiling only
 // changes which threads and blocks do a line's butterflies, never the
 // arithmetic, so every choice gives bit-identical results.
 NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, int n,
@@ -2509,7 +2512,18 @@ NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     };
     NLTune out;
     float bz = 1e30f;
-    const int xb_max = zwave_size(g.Nz) ? 0 : zpass_xb_max(g, nphys);  // the wavefront z pass has no tiling
+    if (zwave_size(g.Nz)) {  // wavefront z pass: wavefronts per block
+        for (int wb = 1; wb <= 4; wb *= 2) {
+            NLTune c;
+            c.zwb = wb;
+            const float ms = timed([&] { launch_z_any(method, g, pz, n, tab, want_g, xp, zo, sx, c, st); });
+            if (ms < bz) {
+                bz = ms;
+                out.zwb = wb;
+            }
+        }
+    }
+    const int xb_max = zwave_size(g.Nz) ? 0 : zpass_xb_max(g, nphys);
     for (int xb = 1; xb <= xb_max; xb *= 2) {
         NLTune c;
         c.zxb = xb;
@@ -2520,7 +2534,19 @@ NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
         }
     }
     float bx = 1e30f;
-    const int xl_max = zwave_size(g.Nx) ? 0 : xfft_lines_max(g);  // the wavefront x-FFTs have no tiling
+    if (zwave_size(g.Nx)) {  // wavefront x-FFTs: wavefronts per block, timed on the inverse
+        for (int wb = 1; wb <= 4; wb *= 2) {
+            NLTune c;
+            c.xwb = wb;
+            const float ms = timed([&] { launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, n, yp,
+                                                     sy, xp, sx, c, st); });
+            if (ms < bx) {
+                bx = ms;
+                out.xwb = wb;
+            }
+        }
+    }
+    const int xl_max = zwave_size(g.Nx) ? 0 : xfft_lines_max(g);
     for (int xl = 1; xl <= xl_max; xl *= 2) {
         NLTune c;
         c.xl = xl;

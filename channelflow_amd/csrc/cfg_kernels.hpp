@@ -209,8 +209,10 @@ inline long nlwork_x_doubles(const Geom& g) { return 2L * 9 * g.Ny * g.Nx * g.nk
 
 // Tile shapes of the nonlinear-term launches (0: the default rule).
 struct NLTune {
-    int zxb = 0;  // (y, x) lines per block of the z-pass
-    int xl = 0;   // kz lines per block of the x-FFTs
+    int zxb = 0;  // (y, x) lines per block of the LDS z-pass
+    int xl = 0;   // kz lines per block of the LDS x-FFTs
+    int zwb = 0;  // wavefronts per block of the wavefront z-pass (0: 4)
+    int xwb = 0;  // wavefronts per block of the wavefront x-FFTs (0: 4)
 };
 
 void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, int n,
