@@ -18,6 +18,10 @@ The same JSON line also carries
               horizons 100-2000 steps, admitted by HBM budget, with p50/p99
               completion latency and aggregate grid-point*steps/s, beside the
               same requests run one at a time on the plain path;
+  multishooting one call of the multiple-shooting batch client: the residual
+              of 16 shots and 8 Jacobian products, 144 integrations of
+              32x49x32 over T = 4 in one Batch, timed whole, beside the same
+              integrations on the plain path;
   single_c2   the plain 192x129x192 Poiseuille channel (rotational, SBDF3,
               dt = 0.005) and its per-kernel times;
   graphs      hipGraph captures and replays inside the timed region;
@@ -79,6 +83,45 @@ def c4_members(seed=2024):
             out.append(dict(grid=(32, 49, 32), flags=COUETTE, norm=norm, decay=0.3, seed=seed, member=b,
                             delta=(seed + 7919, 1000 * b + k)))
     return out
+
+
+def run_multishooting(cfa, T=4.0, nshot=16, ndir=8, seed=2024, plain_sample=9):
+    """One call of the multiple-shooting batch client (channelflow_amd.dsi):
+    G(x) of 16 shots plus 8 forward-difference Jacobian products at
+    epsDx = 1e-7, i.e. 16 x (1 + 8) = 144 integrations of 32x49x32 over T,
+    in one Batch. Timed whole (member set-up, tau data, autotuning, the steps,
+    results copied back), as a Newton-Krylov iteration would pay it. The plain
+    path runs `plain_sample` of the same integrations one DNS at a time (one
+    shot and its 8 perturbed copies) and is scaled to 144."""
+    from channelflow_amd.dsi import MultiShooting
+    from channelflow_amd.fields import random_field
+    Nx, Ny, Nz = 32, 49, 32
+    shots = [member_field(m) for m in c4_members(seed) if "delta" not in m][:nshot]
+    dirs = [[random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=1.0, decay=0.3, seed=seed + 7919, member=1000 * b + k)
+             for b in range(nshot)] for k in range(ndir)]
+    fl = cfa.DNSFlags(**dict(COUETTE, variabledt=False))
+    ms = MultiShooting(Nx, Ny, Nz, LX_C, LZ_C, fl, T)
+    t0 = time.perf_counter()
+    G, DG, eps = ms.eval_with_jacobians(shots, dirs, epsDx=1e-7)
+    wall = time.perf_counter() - t0
+    n_int = nshot * (1 + ndir)
+    pts = Nx * Ny * Nz
+    sample = [shots[0]] + [shots[0] + eps[k] * dirs[k][0] for k in range(ndir)]
+    sample = sample[:plain_sample]
+    t0 = time.perf_counter()
+    for u in sample:
+        d = cfa.DNS(flow_fields(cfa, (Nx, Ny, Nz), u), ms.flags)
+        d.advance_resident(ms.nsteps)
+        d.state()
+    wall_p = (time.perf_counter() - t0) * n_int / len(sample)
+    gnorm = math.sqrt(sum(float(np.vdot(g, g).real) for g in G))
+    return {"workload": f"C4 multi-shooting call: {nshot} shots x (1 + {ndir} Jacobian directions) = {n_int} "
+                        f"plane Couette Re=400 integrations at {Nx}x{Ny}x{Nz}, T={T} ({ms.nsteps} SBDF3 steps, "
+                        f"dt={ms.flags.dt}), epsDx=1e-7",
+            "integrations": n_int, "seconds": wall, "value": n_int * pts * ms.nsteps / wall,
+            "unit": "grid-pt*steps/s", "plain_gpu_seconds": wall_p,
+            "plain_sample": f"{len(sample)} of the {n_int} integrations one DNS at a time, scaled",
+            "batched_over_plain": wall_p / wall, "residual_coefficient_norm": gnorm}
 
 
 def member_field(mem):
@@ -343,6 +386,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-ms", action="store_true", help="skip the multiple-shooting call (C4)")
     ap.add_argument("--c2-only", action="store_true",
                     help="run only the plain 192x129x192 channel and print its line (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=96)
@@ -419,6 +463,10 @@ def main():
             sp = plain_for_requests(cfa, args.serve_requests, 4242, 20, 5)
             out["served"]["plain_gpu_same_requests"] = sp
             out["served"]["served_over_plain"] = out["served"]["value"] / sp["value"]
+    if not args.no_ms:
+        print("bench: multiple-shooting call (C4)", file=sys.stderr, flush=True)
+        out["multishooting"] = run_multishooting(cfa)
+        print(f"bench: multishooting {json.dumps(out['multishooting'])}", file=sys.stderr, flush=True)
     if not args.no_c2:
         print("bench: single 192x129x192", file=sys.stderr, flush=True)
         out["single_c2"] = run_single_c2(cfa, 50, 10)

@@ -221,6 +221,7 @@ struct Geometry {
     FFTTable fx, fz;
     DBuf T, TD, Ti, DTi;
     DBuf HB;  // rows of the Chebyshev-tau operator B of the Helmholtz systems, [3][Ny]
+    std::vector<long double> tcheb;  // T_k(y_n) = cos(pi n k / (Ny-1)), [n][k], for host profile evaluation
 
     YMats ymats() const { return YMats{T.p, TD.p, Ti.p, DTi.p}; }
 
@@ -246,6 +247,7 @@ struct Geometry {
         std::vector<long double> t(N * N), d(N * N, 0.0L), ti(N * N);
         for (int y = 0; y < N; ++y)
             for (int k = 0; k < N; ++k) t[y * N + k] = cos_pi_frac((long)y * k, N - 1);
+        tcheb = t;
         for (int k = 0; k < N; ++k)
             for (int y = 0; y < N; ++y) {
                 const long double sk = (k == 0 || k == N - 1) ? 0.5L : 1.0L;
@@ -299,7 +301,7 @@ struct Geometry {
         std::vector<double> v(N);
         for (int y = 0; y < N; ++y) {
             long double s = 0.0L;
-            for (int k = 0; k < N; ++k) s += cos_pi_frac((long)y * k, N - 1) * (long double)cd[k];
+            for (int k = 0; k < N; ++k) s += tcheb[(size_t)y * N + k] * (long double)cd[k];
             v[y] = (double)s;
         }
         return v;
@@ -414,18 +416,27 @@ class DevPool {
     DevPool(const DevPool&) = delete;
     DevPool& operator=(const DevPool&) = delete;
     ~DevPool() {
-        for (auto& kv : free_)
-            for (double* p : kv.second) (void)hipFree(p);
+        for (double* p : slabs_) (void)hipFree(p);
     }
+    // A buffer of n doubles. An empty size class gets a slab of several
+    // buffers from one hipMalloc: the count doubles with every slab of the
+    // class (capped at 256 MB a slab), so m buffers of one size cost
+    // O(log m) allocations.
     double* take(size_t n) {
-        auto it = free_.find(n);
-        if (it != free_.end() && !it->second.empty()) {
-            double* p = it->second.back();
-            it->second.pop_back();
-            return p;
+        auto& fl = free_[n];
+        if (fl.empty()) {
+            size_t& made = made_[n];
+            const size_t bytes = n * sizeof(double);
+            size_t pieces = std::max<size_t>(1, made);
+            pieces = std::min<size_t>(pieces, std::max<size_t>(1, (size_t(256) << 20) / std::max<size_t>(bytes, 1)));
+            double* slab = nullptr;
+            HIPCHK(hipMalloc(&slab, pieces * bytes));
+            slabs_.push_back(slab);
+            for (size_t i = pieces; i-- > 0;) fl.push_back(slab + i * n);
+            made += pieces;
         }
-        double* p = nullptr;
-        HIPCHK(hipMalloc(&p, n * sizeof(double)));
+        double* p = fl.back();
+        fl.pop_back();
         return p;
     }
     void give(double* p, size_t n) {
@@ -434,7 +445,27 @@ class DevPool {
 
    private:
     std::map<size_t, std::vector<double*>> free_;
+    std::map<size_t, size_t> made_;  // buffers handed out per size class so far
+    std::vector<double*> slabs_;
 };
+
+// Device copy of a small constant table, shared by every user with the same
+// contents (the base-flow constants of the members of one grid, base flow and
+// nu). Expired entries are swept as the cache grows.
+inline std::shared_ptr<DBuf> shared_consts(const std::vector<double>& h) {
+    static std::map<std::vector<double>, std::weak_ptr<DBuf>> cache;
+    auto it = cache.find(h);
+    if (it != cache.end()) {
+        if (auto sp = it->second.lock()) return sp;
+        cache.erase(it);
+    }
+    if (cache.size() >= 256)
+        for (auto jt = cache.begin(); jt != cache.end();) jt = jt->second.expired() ? cache.erase(jt) : std::next(jt);
+    auto sp = std::make_shared<DBuf>(h.size());
+    HIPCHK(hipMemcpy(sp->p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    cache[h] = sp;
+    return sp;
+}
 
 // A device buffer taken from a pool
 struct PBuf {
@@ -634,8 +665,7 @@ class FlowModel {
             std::copy(der.begin(), der.end(), dev.begin() + off + Ny);
             off += 2 * Ny;
         }
-        consts_.alloc(6 * Ny);
-        HIPCHK(hipMemcpy(consts_.p, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice));
+        consts_ = shared_consts(dev);
         const std::vector<double> Uy = host_diff(Ubase_, L), Wy = host_diff(Wbase_, L);
         nuUslope_ = flags_.nu * (host_eval_b(Uy) - host_eval_a(Uy)) / L;
         nuWslope_ = flags_.nu * (host_eval_b(Wy) - host_eval_a(Wy)) / L;
@@ -645,17 +675,17 @@ class FlowModel {
     const cfg_flags& flags() const { return flags_; }
     const std::vector<double>& Ubase() const { return Ubase_; }
     const std::vector<double>& Wbase() const { return Wbase_; }
-    const double* base_values() const { return consts_.p + 2 * Ny_; }  // [4][Ny]: U, U', W, W' at y_n
+    const double* base_values() const { return consts_->p + 2 * Ny_; }  // [4][Ny]: U, U', W, W' at y_n
     double UbulkAct() const { return UbulkAct_; }
     double dPdxAct0() const { return dPdxAct_; }
     double dPdzAct0() const { return dPdzAct_; }
 
     SolveConst solve_const(double* dP) const {
-        return SolveConst{consts_.p, consts_.p + Ny_, dPdxRef_, dPdzRef_, UbulkRef_ - UbulkBase_, WbulkRef_ - WbulkBase_,
+        return SolveConst{consts_->p, consts_->p + Ny_, dPdxRef_, dPdzRef_, UbulkRef_ - UbulkBase_, WbulkRef_ - WbulkBase_,
                           flags_.constraint == CFG_BulkVelocity ? 1 : 0, 0, dP, dP + 1};
     }
     LinearConst linear_const() const {
-        return LinearConst{consts_.p, consts_.p + Ny_, dPdxRef_, dPdzRef_, flags_.constraint == CFG_BulkVelocity ? 1 : 0,
+        return LinearConst{consts_->p, consts_->p + Ny_, dPdxRef_, dPdzRef_, flags_.constraint == CFG_BulkVelocity ? 1 : 0,
                            0, nuUslope_, nuWslope_};
     }
 
@@ -725,26 +755,45 @@ class FlowModel {
     cfg_flags flags_;
     int Ny_ = 0;
     std::vector<double> Ubase_, Wbase_;
-    DBuf consts_;  // [6][Ny]: nu U'', nu W'' (coefficients), then U, U', W, W' at the y_n
+    std::shared_ptr<DBuf> consts_;  // [6][Ny]: nu U'', nu W'' (coefficients), then U, U', W, W' at the y_n
     double nuUslope_ = 0, nuWslope_ = 0;
     double dPdxRef_ = 0, dPdxAct_ = 0, dPdzRef_ = 0, dPdzAct_ = 0, UbulkRef_ = 0, UbulkAct_ = 0, UbulkBase_ = 0;
     double WbulkRef_ = 0, WbulkAct_ = 0, WbulkBase_ = 0;
 };
+
+// Pinned host buffer that host <-> device field copies pass through (copies
+// from pageable memory are staged by the runtime in small pieces). Grows as
+// needed; one per process, used by one copy at a time.
+inline double* pinned_stage(size_t n) {
+    static double* buf = nullptr;
+    static size_t cap = 0;
+    if (n > cap) {
+        if (buf) (void)hipHostFree(buf);
+        buf = nullptr;
+        cap = 0;
+        HIPCHK(hipHostMalloc((void**)&buf, n * sizeof(double), hipHostMallocDefault));
+        cap = n;
+    }
+    return buf;
+}
 
 inline std::vector<double> upload_packed(const Geom& g, int nd, const double* ref, double* dst, hipStream_t st) {
     std::vector<double> h;
     pack(g, nd, ref, h);
     for (double v : h)
         if (!std::isfinite(v)) throw Error(CFG_ERR_NONFINITE, "non-finite value in the input field");
-    HIPCHK(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    double* pin = pinned_stage(h.size());
+    std::memcpy(pin, h.data(), h.size() * sizeof(double));
+    HIPCHK(hipMemcpyAsync(dst, pin, h.size() * sizeof(double), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     return h;
 }
 
 inline void download_packed(const Geom& g, int nd, const double* src, size_t n, double* ref, hipStream_t st) {
-    std::vector<double> h(n);
-    HIPCHK(hipMemcpyAsync(h.data(), src, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    double* pin = pinned_stage(n);
+    HIPCHK(hipMemcpyAsync(pin, src, n * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    std::vector<double> h(pin, pin + n);
     unpack(g, nd, h, ref);
 }
 
