@@ -26,8 +26,10 @@ The same JSON line also carries
               dt = 0.005) and its per-kernel times;
   graphs      hipGraph captures and replays inside the timed region;
   roofline    the dominant kernel of the headline workload (HIP events over
-              a profiled pass of the same ensemble right after the timed
-              region; replay is off while events are recorded);
+              a profiled pass of the same members right after the timed
+              region, one grid class at a time so that a launch's events
+              time that kernel alone; replay is off while events are
+              recorded);
   cpu_baseline the oracle (oracle/cpu_ref.cpp, one core) on a bounded sample.
 """
 import argparse
@@ -174,8 +176,11 @@ def kernel_bytes(grid, method):
 def run_batched(cfa, members, steps, warmup, prof_steps):
     """Timed region: `steps` steps of the whole ensemble with each class's
     repeating ticks replayed as hipGraphs and no events between launches.
-    Then `prof_steps` more steps with HIP events around every launch (graphs
-    off while events are on) give the per-kernel durations."""
+    Then the per-kernel durations: each grid class alone (its members in a
+    Batch of their own, warmed up), `prof_steps` steps with HIP events around
+    every launch (graphs off while events are on). One class at a time, so a
+    launch's events time that kernel, not the other classes' kernels running
+    beside it on their streams; the sums over the classes are returned."""
     batch = cfa.Batch()
     for mem in members:
         batch.add(flow_fields(cfa, mem["grid"], member_field(mem)), cfa.DNSFlags(**mem["flags"]))
@@ -185,10 +190,22 @@ def run_batched(cfa, members, steps, warmup, prof_steps):
     batch.advance(steps)                  # returns after every stream has drained
     wall = time.perf_counter() - t0
     c1, r1 = batch.graph_stats()
-    batch.profile(True)
-    batch.advance(prof_steps)
-    kt = batch.kernel_times()
-    batch.profile(False)
+    kt = {}
+    for g in sorted({m["grid"] for m in members}):
+        cls = cfa.Batch()
+        for mem in members:
+            if mem["grid"] == g:
+                cls.add(flow_fields(cfa, mem["grid"], member_field(mem)), cfa.DNSFlags(**mem["flags"]))
+        cls.advance(warmup)
+        cls.profile(True)
+        cls.advance(prof_steps)
+        for k, (ms, n) in cls.kernel_times().items():
+            a = kt.setdefault(k, [0.0, 0])
+            a[0] += ms
+            a[1] += n
+        cls.profile(False)
+        del cls
+    kt = {k: tuple(v) for k, v in kt.items()}
     for mid in range(0, len(members), max(1, len(members) // 8)):
         u, _ = batch.state(mid)
         if not np.all(np.isfinite(u)):
@@ -358,8 +375,9 @@ def cpu_baseline(members, steps_timed=2, budget_s=25.0, min_s=5.0):
 
 def roofline(members, kt, steps):
     """Dominant kernel of the batched run: algorithmic bytes of its launches
-    over `steps` profiled steps / their summed HIP-event durations (events
-    recorded on each class's own stream around every launch)."""
+    over `steps` profiled steps of every grid class / their summed HIP-event
+    durations (events recorded on the class's stream around every launch,
+    one class at a time)."""
     totals = {}
     for mem in members:
         for k, b in kernel_bytes(mem["grid"], mem["flags"]["nonlinearity"]).items():
