@@ -17,7 +17,8 @@ The same JSON line also carries
               Poiseuille (Re_bulk 1500-3000) requests on grids 32^3 ... 128^3,
               horizons 100-2000 steps, admitted by HBM budget, with p50/p99
               completion latency and aggregate grid-point*steps/s, beside the
-              same requests run one at a time on the plain path;
+              same requests run one at a time on the plain path; `saturated`
+              repeats it with every request queued at t = 0 (the capacity);
   multishooting one call of the multiple-shooting batch client: the residual
               of 16 shots and 8 Jacobian products, 144 integrations of
               32x49x32 over T = 4 in one Batch, timed whole, beside the same
@@ -255,8 +256,14 @@ def serve_requests(n, seed):
 
 
 def run_served(cfa, n, rate, seed, budget):
+    """The request stream through a Server: Poisson arrivals at `rate` per
+    second, or, with rate None, every request queued at t = 0 (the server's
+    capacity, with latency then counting the time spent queued)."""
     reqs = serve_requests(n, seed)
-    arrivals = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, n))
+    if rate is None:
+        arrivals = np.zeros(n)
+    else:
+        arrivals = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / rate, n))
     fields = [flow_fields(cfa, q["grid"], member_field(q)) for q in reqs]
     srv = cfa.Server(budget)
     rid_of = {}
@@ -476,11 +483,17 @@ def main():
         print("bench: serving", file=sys.stderr, flush=True)
         out["served"] = run_served(cfa, args.serve_requests, args.serve_rate, 4242, args.serve_budget)
         print(f"bench: served {json.dumps(out['served'])}", file=sys.stderr, flush=True)
+        print("bench: serving, every request queued at t=0", file=sys.stderr, flush=True)
+        sat = run_served(cfa, args.serve_requests, None, 4242, args.serve_budget)
+        sat["arrival_rate_per_s"] = "all at t=0"
+        out["served"]["saturated"] = sat
+        print(f"bench: served saturated {json.dumps(sat)}", file=sys.stderr, flush=True)
         if not args.no_plain:
             print("bench: served request mix on the plain path", file=sys.stderr, flush=True)
             sp = plain_for_requests(cfa, args.serve_requests, 4242, 20, 5)
             out["served"]["plain_gpu_same_requests"] = sp
             out["served"]["served_over_plain"] = out["served"]["value"] / sp["value"]
+            sat["served_over_plain"] = sat["value"] / sp["value"]
     if not args.no_ms:
         print("bench: multiple-shooting call (C4)", file=sys.stderr, flush=True)
         out["multishooting"] = run_multishooting(cfa)
