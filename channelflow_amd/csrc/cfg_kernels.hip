@@ -1947,8 +1947,9 @@ __device__ __forceinline__ mfma4d mfma16x16x4(double a, double b, mfma4d c) {
 
 // Waves per SIMD the register allocation of a wave-column kernel must allow:
 // the accumulators take 16 NRT registers, the rest is operand staging (without
-// a bound the scheduler hoists a whole chunk's LDS reads).
-constexpr int yw_waves(int nrt) { return nrt <= 4 ? 4 : (nrt <= 6 ? 3 : 2); }
+// a bound the scheduler hoists a whole chunk's LDS reads) and, in the inverse,
+// the next matrix chunk in flight.
+constexpr int yw_waves(int nrt) { return nrt <= 3 ? 4 : (nrt <= 4 ? 3 : 2); }
 // The forward transform also holds sums and differences of mirrored values of
 // the prefetched chunk.
 constexpr int yw_waves_fwd(int nrt) { return nrt <= 1 ? 4 : (nrt <= 3 ? 3 : 2); }
@@ -1967,8 +1968,11 @@ __global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((a
     constexpr int NMAT = (METHOD == NL_DIV) ? 1 : 2, NT = 192 * NMAT;
     constexpr int KC = 16, AP = KC + 1, EP = 17;  // chunk columns: 8 even k, then 8 odd k
     constexpr int RS = NRT * 16 + 1;              // LDS rows per matrix: tiles, middle point
+    // two matrix chunk buffers: chunk i + 1 is loaded into registers while
+    // chunk i is multiplied, and stored to the other buffer after it
     constexpr int SA = NMAT * RS * AP, SE = NMAT * 3 * 2 * 16 * EP;
-    __shared__ double sm[SA > SE ? SA : SE];
+    constexpr int NSTG = NMAT * RS * KC, NST = (NSTG + NT - 1) / NT;  // staged entries per chunk, per thread
+    __shared__ double sm[2 * SA > SE ? 2 * SA : SE];
     const NLMember& me = tab[blockIdx.y];
     double2* __restrict__ yp = yp0 + blockIdx.y * sy;
     const int Ny = g.Ny, nm = Ny / 2, mid = (Ny & 1) ? nm : -1;
@@ -1979,8 +1983,6 @@ __global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((a
     const long mcol = m0 + (lr >> 1);
     const bool mok = mcol < M;
     const double* __restrict__ ucol = me.u + 2 * ((long)c * Ny * M + mcol) + (l & 1);
-    const double* __restrict__ arow = sm + (am * RS + lr) * AP + lk;
-    const double* __restrict__ amid = sm + (am * RS + NRT * 16) * AP + lk;
     mfma4d ae[NRT], ao[NRT];
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt) ae[rt] = ao[rt] = mfma4d{0.0, 0.0, 0.0, 0.0};
@@ -1993,20 +1995,41 @@ __global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((a
             bn[i] = (mok && k < Ny) ? ucol[2L * k * M] : 0.0;
         }
     };
+    // matrix entries of chunk k0: entry i = (matrix a, row slot, column kk),
+    // stored with the 8 even columns before the 8 odd ones
+    double mv[NST];
+    auto load_mat = [&](int k0) {
+#pragma unroll
+        for (int s = 0; s < NST; ++s) {
+            const int i = threadIdx.x + s * NT;
+            const int kk = i % KC, slot = (i / KC) % RS, a = i / (KC * RS);
+            const int k = k0 + kk, row = slot < NRT * 16 ? slot : mid;
+            const bool ok = i < NSTG && (slot < NRT * 16 ? slot < nm : mid >= 0) && k < Ny;
+            mv[s] = ok ? (a == 0 ? ym.T : ym.TD)[(long)row * Ny + k] : 0.0;
+        }
+    };
     fetch(0);
-    for (int k0 = 0; k0 < Ny; k0 += KC) {
+    load_mat(0);
+    for (int k0 = 0, buf = 0; k0 < Ny; k0 += KC, buf ^= 1) {
         double b[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) b[i] = bn[i];
-        __syncthreads();  // the previous chunk's matrix reads are done
-        for (int i = threadIdx.x; i < NMAT * RS * KC; i += NT) {
+        // buffer buf was last read two chunks ago, before every thread passed
+        // the previous chunk's barrier
+        double* __restrict__ sb = sm + buf * SA;
+#pragma unroll
+        for (int s = 0; s < NST; ++s) {
+            const int i = threadIdx.x + s * NT;
             const int kk = i % KC, slot = (i / KC) % RS, a = i / (KC * RS);
-            const int k = k0 + kk, row = slot < NRT * 16 ? slot : mid;
-            const bool ok = (slot < NRT * 16 ? slot < nm : mid >= 0) && k < Ny;
-            sm[(a * RS + slot) * AP + (kk & 1) * 8 + (kk >> 1)] = ok ? (a == 0 ? ym.T : ym.TD)[(long)row * Ny + k] : 0.0;
+            if (i < NSTG) sb[(a * RS + slot) * AP + (kk & 1) * 8 + (kk >> 1)] = mv[s];
         }
-        if (k0 + KC < Ny) fetch(k0 + KC);
         __syncthreads();
+        if (k0 + KC < Ny) {
+            fetch(k0 + KC);
+            load_mat(k0 + KC);
+        }
+        const double* __restrict__ arow = sb + (am * RS + lr) * AP + lk;
+        const double* __restrict__ amid = sb + (am * RS + NRT * 16) * AP + lk;
 #pragma unroll
         for (int st = 0; st < 2; ++st) {
             // unconditional: staged rows and columns past the grid are zero,
