@@ -1951,8 +1951,9 @@ __device__ __forceinline__ mfma4d mfma16x16x4(double a, double b, mfma4d c) {
 // the next matrix chunk in flight.
 constexpr int yw_waves(int nrt) { return nrt <= 3 ? 4 : (nrt <= 4 ? 3 : 2); }
 // The forward transform also holds sums and differences of mirrored values of
-// the prefetched chunk.
-constexpr int yw_waves_fwd(int nrt) { return nrt <= 1 ? 4 : (nrt <= 3 ? 3 : 2); }
+// the prefetched chunk, and the next matrix chunk in flight (two matrices in
+// the divergence forms).
+constexpr int yw_waves_fwd(bool divf, int nrt) { return nrt <= 1 ? 4 : (nrt <= (divf ? 2 : 3) ? 3 : 2); }
 
 // Chebyshev -> y values (outputs as k_yinv_t): p = T u and, unless the form is
 // the divergence one, q = T D u = du/dy values. Waves 0-2 compute p of
@@ -2119,13 +2120,18 @@ __global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((a
 // is a dot product per lane, summed over the lane groups. NRT: row tiles of 16
 // coefficients of each parity.
 template <int METHOD, int NRT>
-__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_waves_fwd(NRT)))) k_yfwd_w(Geom g, YMats ym, int have_g, const double2* __restrict__ fo0,
+__global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_waves_fwd(METHOD == NL_DIV || METHOD == NL_SKEW, NRT)))) k_yfwd_w(Geom g, YMats ym, int have_g, const double2* __restrict__ fo0,
                                                 long sy, const NLMember* __restrict__ tab) {
     constexpr bool DIVF = (METHOD == NL_DIV || METHOD == NL_SKEW);
     constexpr int NMAT = DIVF ? 2 : 1;
     constexpr int KC = 8, AP = KC + 1, NS = KC / 4;
     constexpr int RS = 2 * NRT * 16 + 1;  // LDS rows per matrix: even, odd coefficients, last one
-    __shared__ double sm[NMAT * RS * AP];
+    // two matrix chunk buffers, as in k_yinv_w, unless the chunk in flight
+    // would not fit the registers (two matrices of 6 or 8 row tiles): then one
+    // buffer, loaded and stored between two barriers
+    constexpr int SA = NMAT * RS * AP, NSTG = NMAT * RS * KC, NST = (NSTG + 191) / 192;
+    constexpr bool DB = !(DIVF && NRT >= 6);
+    __shared__ double sm[(DB ? 2 : 1) * SA];
     const double* __restrict__ fo = reinterpret_cast<const double*>(fo0 + blockIdx.y * sy);
     double* __restrict__ f = tab[blockIdx.y].f;
     const int Ny = g.Ny, nh = (Ny + 1) / 2, nm = Ny / 2;
@@ -2178,8 +2184,22 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
             dhn[s] = H - Hm;
         }
     };
+    // matrix entries of chunk y0: entry i = (matrix a, row slot, column kk)
+    double mv[NST];
+    auto load_mat = [&](int y0) {
+#pragma unroll
+        for (int s = 0; s < NST; ++s) {
+            const int i = threadIdx.x + s * 192;
+            const int kk = i % KC, slot = (i / KC) % RS, a = i / (KC * RS);
+            const int par = slot / (NRT * 16), r = slot % (NRT * 16), y = y0 + kk;  // par 2: the last coefficient
+            const int k = par < 2 ? 2 * r + par : Ny - 1;
+            const bool ok = i < NSTG && (par < 2 ? r < nm : odd) && y < nh;
+            mv[s] = ok ? (a == 0 ? ym.Ti : ym.DTi)[(long)k * Ny + y] : 0.0;
+        }
+    };
     fetch(0);
-    for (int y0 = 0; y0 < nh; y0 += KC) {
+    if (DB) load_mat(0);
+    for (int y0 = 0, buf = 0; y0 < nh; y0 += KC, buf ^= DB) {
         double sg[NS], dg[NS], sh[NS], dh[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
@@ -2188,23 +2208,38 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
             sh[s] = shn[s];
             dh[s] = dhn[s];
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < NMAT * RS * KC; i += 192) {
-            const int kk = i % KC, slot = (i / KC) % RS, a = i / (KC * RS);
-            const int par = slot / (NRT * 16), r = slot % (NRT * 16), y = y0 + kk;  // par 2: the last coefficient
-            const int k = par < 2 ? 2 * r + par : Ny - 1;
-            const bool ok = (par < 2 ? r < nm : odd) && y < nh;
-            sm[(a * RS + slot) * AP + kk] = ok ? (a == 0 ? ym.Ti : ym.DTi)[(long)k * Ny + y] : 0.0;
+        // DB: buffer buf was last read two chunks ago, before every thread
+        // passed the previous chunk's barrier
+        double* __restrict__ sb = sm + buf * SA;
+        if (DB) {
+#pragma unroll
+            for (int s = 0; s < NST; ++s) {
+                const int i = threadIdx.x + s * 192;
+                const int kk = i % KC, slot = (i / KC) % RS, a = i / (KC * RS);
+                if (i < NSTG) sb[(a * RS + slot) * AP + kk] = mv[s];
+            }
+        } else {
+            __syncthreads();  // the previous chunk's matrix reads are done
+            for (int i = threadIdx.x; i < NSTG; i += 192) {
+                const int kk = i % KC, slot = (i / KC) % RS, a = i / (KC * RS);
+                const int par = slot / (NRT * 16), r = slot % (NRT * 16), y = y0 + kk;
+                const int k = par < 2 ? 2 * r + par : Ny - 1;
+                const bool ok = (par < 2 ? r < nm : odd) && y < nh;
+                sb[(a * RS + slot) * AP + kk] = ok ? (a == 0 ? ym.Ti : ym.DTi)[(long)k * Ny + y] : 0.0;
+            }
         }
-        if (y0 + KC < nh) fetch(y0 + KC);
         __syncthreads();
+        if (y0 + KC < nh) {
+            fetch(y0 + KC);
+            if (DB) load_mat(y0 + KC);
+        }
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
             // unconditional: staged rows and columns past the grid are zero,
             // and a conditional update keeps the tiles from staying in place
 #pragma unroll
             for (int rt = 0; rt < NRT; ++rt) {
-                const double* A = sm + (rt * 16 + lr) * AP + 4 * s + lk;
+                const double* A = sb + (rt * 16 + lr) * AP + 4 * s + lk;
                 ae[rt] = mfma16x16x4(A[0], sg[s], ae[rt]);
                 ao[rt] = mfma16x16x4(A[NRT * 16 * AP], dg[s], ao[rt]);
                 if (DIVF) {
@@ -2212,7 +2247,7 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
                     ao[rt] = mfma16x16x4(A[(RS + NRT * 16) * AP], sh[s], ao[rt]);
                 }
             }
-            const double* A = sm + 2 * NRT * 16 * AP + 4 * s + lk;
+            const double* A = sb + 2 * NRT * 16 * AP + 4 * s + lk;
             dlast = fma(A[0], sg[s], dlast);
             if (DIVF) dlast = fma(A[RS * AP], dh[s], dlast);
             __builtin_amdgcn_sched_barrier(0);  // keep each step's operand reads next to its MFMAs
