@@ -424,7 +424,7 @@ def main():
     args = ap.parse_args()
     if args.traffic_from:
         cmd = ("rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE) -f csv -- python bench.py --workload "
-               f"{args.workload} --steps 5 --warmup 3 --no-cpu --no-serve --no-c2 --no-plain --copy-calibration")
+               f"{args.workload} --steps 5 --warmup 3 --no-cpu --no-serve --no-c2 --no-plain --no-ms --copy-calibration")
         traffic_from_counters(*args.traffic_from, os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json"),
                               cmd)
         return
