@@ -67,6 +67,29 @@ def test_nonlinear_single_call(gpu, method, dealias, grid):
     assert np.all(err <= 1e-12), err
 
 
+@pytest.mark.parametrize("method", ["Rotational", "Convection", "Divergence", "SkewSymmetric"])
+@pytest.mark.parametrize("Ny", [33, 49, 65, 97, 129, 161, 193, 257, 289])
+def test_nonlinear_chebyshev_sizes(gpu, method, Ny):
+    """y transforms at every row-tile count of the matrix-core kernels (16
+    upper points or coefficient pairs per tile: Ny = 33 one tile, 49 and 65
+    two, 97 three, 129 four, 161 and 193 six, 257 eight), including the
+    chunk-wise LDS staging of the forward transform's divergence forms at six
+    and eight tiles, and the vector-tile kernels above 257 (Ny = 289). A
+    single nonlinear-term call against the oracle."""
+    Nx, Nz = 8, 8
+    g = R.Grid(Nx, Ny, Nz, LX_C, LZ_C)
+    u = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.3, decay=0.5, seed=13, dealias=True)
+    fl = dict(nonlinearity=method, dealiasing="DealiasXZ", baseflow="LaminarBase", uupperwall=1.0,
+              ulowerwall=-1.0, constraint="BulkVelocity", nu=1 / 400, Vsuck=0.0)
+    f_ref, _ = R.nonlinear(g, R.make_flags(**fl), u)
+    f_gpu = cfa.nonlinear(cfa.FlowField(Nx, Ny, Nz, 3, LX_C, LZ_C, data=u), cfa.DNSFlags(**fl))
+    m = live_mask(Nx, Nz, Nx // 3 - 1, Nz // 3 - 1)
+    fr = np.where(m[None, None], f_ref, 0)
+    err = rel_l2_per_component(g, f_gpu, fr)
+    print(Ny, method, "rel L2 per component", err, flush=True)
+    assert np.all(err <= 1e-12), err
+
+
 @pytest.mark.parametrize("constraint", ["PressureGradient", "BulkVelocity"])
 @pytest.mark.parametrize("taucorr", [True, False])
 @pytest.mark.parametrize("grid", [(16, 33, 16), (8, 25, 8), (8, 65, 8), (8, 97, 8), (8, 129, 8), (6, 193, 6)])
