@@ -673,12 +673,25 @@ __global__ void __launch_bounds__(W * MPB) k_pressure_wave(Geom g, YMats ym, Pre
     }
 }
 
+// Raises a kernel's dynamic-LDS limit once per (kernel, size), so that launches
+// recorded into a graph make no attribute calls.
+template <class K>
+static void allow_lds(K kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return;
+    static std::map<const void*, size_t> done;
+    size_t& d = done[(const void*)kernel];
+    if (d >= bytes) return;
+    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    d = bytes;
+}
+
 void launch_pressure(const Geom& g, const YMats& ym, const double* hb, const PressArgs& pa, hipStream_t st) {
     const int Me = (g.Ny + 1) / 2;
     auto go = [&](auto kern, int w) {
         const int mpb = 256 / w;
         const dim3 grid((unsigned)((g.M + mpb - 1) / mpb));
         const size_t lds = (size_t)4 * g.Ny * mpb * sizeof(double2) + (size_t)3 * g.Ny * sizeof(double);
+        allow_lds(kern, lds);
         kern<<<grid, 256, lds, st>>>(g, ym, pa, hb);
     };
     if (Me <= 16) go(k_pressure_wave<1, 16, 16>, 16);
@@ -696,6 +709,7 @@ void launch_tau_solve(const Geom& g, const double* hb, int n, const TauMember* t
         const int mpb = 256 / w;
         const dim3 grid((unsigned)((g.M + mpb - 1) / mpb), n);
         const size_t lds = (size_t)3 * g.Ny * mpb * sizeof(double2) + (size_t)3 * g.Ny * sizeof(double);
+        allow_lds(kern, lds);
         kern<<<grid, 256, lds, st>>>(g, hb, tab);
     };
     // Up to 48 even coefficients: 16-lane segments (four modes per wave, every
@@ -2339,18 +2353,6 @@ static int zpass_xb(const Geom& g, int nphys, const NLTune& tn) {
 static int xfft_lines(const Geom& g, const NLTune& tn) {
     const int m = xfft_lines_max(g);
     return tn.xl > 0 ? std::min(tn.xl, m) : std::min(4, m);
-}
-
-// Raises a kernel's dynamic-LDS limit once per (kernel, size), so that launches
-// recorded into a graph make no attribute calls.
-template <class K>
-static void allow_lds(K kernel, size_t bytes) {
-    if (bytes <= 64 * 1024) return;
-    static std::map<const void*, size_t> done;
-    size_t& d = done[(const void*)kernel];
-    if (d >= bytes) return;
-    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    d = bytes;
 }
 
 #define CFG_FFT_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)

@@ -92,10 +92,12 @@ def test_nonlinear_chebyshev_sizes(gpu, method, Ny):
 
 @pytest.mark.parametrize("constraint", ["PressureGradient", "BulkVelocity"])
 @pytest.mark.parametrize("taucorr", [True, False])
-@pytest.mark.parametrize("grid", [(16, 33, 16), (8, 25, 8), (8, 65, 8), (8, 97, 8), (8, 129, 8), (6, 193, 6)])
+@pytest.mark.parametrize("grid", [(16, 33, 16), (8, 25, 8), (8, 65, 8), (8, 97, 8), (8, 129, 8), (6, 193, 6),
+                                  (6, 257, 6), (6, 321, 6)])
 def test_solve_single_call(gpu, constraint, taucorr, grid):
     """Ny = 25, 33, 65 take 16-lane segments with 1, 2, 3 even coefficients per
-    lane; 97, 129, 193 the 64- and 32-lane segments."""
+    lane; 97, 193, 257 64-lane segments with 1, 2, 3; 129 32-lane segments
+    with 3. Ny = 321 needs more than 64 KB of LDS per block."""
     Nx, Ny, Nz = grid
     g = R.Grid(Nx, Ny, Nz, LX_C, LZ_C)
     rng = np.random.default_rng(5)
@@ -249,6 +251,40 @@ def test_pressure_matches_oracle(gpu, case):
           flush=True)
     assert err <= 1e-12 and err2 <= 1e-12
     assert np.all(fq.data[0][:, ~mask] == 0)
+
+
+@pytest.mark.parametrize("case", ["couette_skew", "poiseuille_rot"])
+@pytest.mark.parametrize("Ny", [49, 97, 193, 257])
+def test_pressure_chebyshev_sizes(gpu, case, Ny):
+    """The pressure kernel's other segment shapes (Ny = 25 above takes 16-lane
+    segments): 32-lane at Ny = 49, 64-lane with 1, 2, 3 coefficients per lane
+    at 97, 193, 257; Ny = 257 needs more than 64 KB of LDS per block.
+
+    The Poisson problem for q loses accuracy about as Ny^4: the oracle's own q
+    moves by 5e-12 (Couette, skew-symmetric, Ny = 97) to 2e-10 (Ny = 257)
+    when its input changes by 1e-15 relative. So the bar is 1e-12 or ten times
+    the oracle's change under two such perturbations, whichever is larger."""
+    flags = PRESSURE_CASES[case]
+    Nx, Nz = 8, 8
+    g = R.Grid(Nx, Ny, Nz, LX_C, LZ_C)
+    u0 = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.2, decay=0.4, seed=52)
+    fl = R.make_flags(**flags)
+    q_ref = R.pressure(g, fl, u0)
+    fu = cfa.FlowField(Nx, Ny, Nz, 3, LX_C, LZ_C, data=u0)
+    fq = cfa.FlowField(Nx, Ny, Nz, 1, LX_C, LZ_C)
+    cfa.PressureSolver(fu, flags=cfa.DNSFlags(**flags)).solve(fq, fu)
+    mask = live_mask(Nx, Nz, Nx // 3 - 1, Nz // 3 - 1)
+    a, b = fq.data[0][:, mask], q_ref[0][:, mask]
+    err = np.linalg.norm(a - b) / np.linalg.norm(b)
+    rng = np.random.default_rng(53)
+    sens = 0.0
+    for _ in range(2):
+        qp = R.pressure(g, fl, u0 * (1 + 1e-15 * rng.standard_normal(u0.shape)))[0][:, mask]
+        sens = max(sens, np.linalg.norm(qp - b) / np.linalg.norm(b))
+    bar = max(1e-12, 10 * sens)
+    print(f"{case} Ny={Ny}: pressure rel L2 vs oracle {err:.3e}, oracle's change under 1e-15 input noise "
+          f"{sens:.3e}, bar {bar:.3e}", flush=True)
+    assert err <= bar
 
 
 def test_golden_trajectory(gpu):
