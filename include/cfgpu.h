@@ -14,6 +14,11 @@
  *
  * Status codes: 0 success, <0 failure; cfg_last_error() describes the last
  * failure on the calling thread.
+ *
+ * Threading: the library keeps process-wide caches (device constants of a
+ * flow model, raised LDS limits, pinned staging buffers), so calls must not
+ * overlap: use it from one host thread at a time. channelflow's DNS is
+ * single-threaded and not re-entrant as well (flowfield.cpp:578-580).
  */
 #ifndef CFGPU_H
 #define CFGPU_H
