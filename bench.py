@@ -570,7 +570,9 @@ def traffic_from_counters(fetch_csv, write_csv, out_path, command):
     scale = {}
     for key, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"), ("write", write_csv, "WRITE_SIZE")):
         rows = _counter_rows(path, counter)
-        cal = [v for n, v in rows if "copyBuffer" in n and v * 1024 > COPY_CAL_BYTES / 2]
+        # the calibration copies, not the small state uploads: FETCH_SIZE reads
+        # about half a copy's bytes, so the cut is at a quarter
+        cal = [v for n, v in rows if "copyBuffer" in n and v * 1024 > COPY_CAL_BYTES / 4]
         if not cal:
             raise SystemExit(f"{path}: no calibration copy dispatches of {COPY_CAL_BYTES} bytes")
         scale[key] = COPY_CAL_BYTES / (1024.0 * sum(cal) / len(cal))

@@ -1,0 +1,84 @@
+"""Host logic of bench.py, no GPU: the one-GPU refusal, the live-mode count
+behind the algorithmic bytes, the serving request stream, and the conversion
+of rocprofv3 counters into HBM bytes per launch."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+@pytest.mark.parametrize("grid", [(32, 33, 32), (48, 49, 48), (64, 65, 64), (192, 129, 192), (12, 13, 18)])
+def test_live_modes_by_enumeration(grid):
+    """live() against a count of the (kx, kz) modes kept by 2/3 dealiasing
+    (|kx| <= Nx/3 - 1, 0 <= kz <= Nz/3 - 1, flowfield.h:578-584) times Ny;
+    the tau solve's bytes are 28 such component arrays."""
+    Nx, Ny, Nz = grid
+    kept = sum(1 for kx in range(-(Nx // 2) + 1, Nx // 2) for kz in range(Nz // 2)
+               if abs(kx) <= Nx // 3 - 1 and kz <= Nz // 3 - 1)
+    assert bench.live(grid) == Ny * kept
+    assert bench.kernel_bytes(grid, "SkewSymmetric")["k_tau_solve"] == 28 * 16 * Ny * kept
+
+
+def test_serve_request_stream():
+    a = bench.serve_requests(96, 4242)
+    assert a == bench.serve_requests(96, 4242)
+    grids = {(32, 33, 32), (48, 49, 48), (64, 65, 64), (96, 97, 96), (128, 129, 128)}
+    assert {q["grid"] for q in a} == grids
+    for q in a:
+        Nx = q["grid"][0]
+        fl = q["flags"]
+        assert 100 <= q["horizon"] <= 2000
+        assert 1e-3 <= q["norm"] <= 0.3
+        if fl["nonlinearity"] == "SkewSymmetric":  # Couette, C1 flags
+            assert 300 <= 1 / fl["nu"] <= 500
+            assert fl["dt"] == pytest.approx(0.02 * 32 / Nx)
+        else:  # Poiseuille, rotational, bulk velocity 1
+            assert fl["nonlinearity"] == "Rotational"
+            assert 1500 <= 1 / fl["nu"] <= 3000
+            assert fl["dPdx"] == pytest.approx(-3 * fl["nu"])
+            assert fl["dt"] == pytest.approx(0.01 * 32 / Nx)
+
+
+def _counters(path, counter, rows):
+    with open(path, "w") as f:
+        f.write("Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value\n")
+        for i, (name, kib) in enumerate(rows):
+            f.write(f'{i},"{name}",{counter},{kib}\n')
+
+
+@pytest.mark.parametrize("fetch_ratio", [0.5, 0.4999, 1.0])
+def test_traffic_from_counters(tmp_path, fetch_ratio):
+    """Counters in KiB per dispatch, scaled by what the same counter reads on
+    the calibration copies (FETCH_SIZE has read about half a copy's bytes);
+    small state-upload copies do not count as calibration, and kernel
+    variants count as their step slot."""
+    cal = bench.COPY_CAL_BYTES / 1024.0
+    tau3 = "void cfg::k_tau_solve_wave<3, 16, 16>(cfg::Geom, double const*, cfg::TauMember const*)"
+    tau2 = "void cfg::k_tau_solve_wave<2, 16, 16>(cfg::Geom, double const*, cfg::TauMember const*)"
+    zw = "void cfg::k_zwave<3, 64>(cfg::Geom, HIP_vector_type<double, 2u> const*)"
+    small = ("__amd_rocclr_copyBuffer", 3.0)
+    fetch = [("__amd_rocclr_copyBuffer", fetch_ratio * cal)] * 3 + [small, (tau3, 100.0), (tau2, 300.0), (zw, 50.0)]
+    write = [("__amd_rocclr_copyBuffer", cal)] * 3 + [small, (tau3, 10.0), (tau2, 30.0), (zw, 20.0)]
+    f, w, out = tmp_path / "f.csv", tmp_path / "w.csv", tmp_path / "t.json"
+    _counters(f, "FETCH_SIZE", fetch)
+    _counters(w, "WRITE_SIZE", write)
+    bench.traffic_from_counters(str(f), str(w), str(out), "test")
+    d = json.load(open(out))
+    sf = 1.0 / fetch_ratio
+    assert d["_scale"]["fetch"] == pytest.approx(sf)
+    assert d["_scale"]["write"] == pytest.approx(1.0)
+    assert d["k_tau_solve"] == pytest.approx(1024 * (sf * 200.0 + 20.0))
+    assert d["k_zpass"] == pytest.approx(1024 * (sf * 50.0 + 20.0))
