@@ -15,6 +15,9 @@ Pins, by reference test:
     (u, v, w, P) satisfying the no-slip BCs -> R -> solve recovers it.
   * dnsParabolaTest (tests/dnsParabolaTest.cpp:60-92, 154-248): the Poiseuille
     parabola is a steady state.
+  * The order of every time-stepping scheme (dnsalgo.cpp:102-179, 351-395,
+    510-576): under dt halving, successive differences of the solution at a
+    fixed time shrink as dt^p with the scheme's own order p.
   * skew = (convection + divergence)/2 and rotational = convection - grad(|u|^2/2)
     identities for the nonlinear term.
   * DNS::CFL (dnsalgo.cpp:63-67, flowfield.cpp:4035-4068): the laminar Couette
@@ -203,6 +206,32 @@ def test_parabola_is_steady(stepping):
     d.advance(10)
     u, _ = d.state()
     assert np.sqrt(R.l2norm2(g, u)) < 1e-13
+
+
+@pytest.mark.parametrize("stepping,init,order", [("SBDF1", "SMRK2", 1), ("SBDF2", "SMRK2", 2), ("SBDF3", "SMRK2", 3),
+                                                  ("SBDF4", "SMRK2", 3), ("CNFE1", "SMRK2", 1), ("CNAB2", "SMRK2", 2),
+                                                  ("SMRK2", "SMRK2", 2), ("CNRK2", "CNRK2", 2), ("SBDF3", "CNRK2", 3)])
+def test_scheme_order(stepping, init, order):
+    """Observed order of convergence at t = 0.64 with dt = T/16 ... T/128: log2
+    of the ratio of successive solution differences. A small field (norm
+    1e-4, no base flow, fixed pressure gradient) keeps the run in the
+    asymptotic regime. SBDF4 shows order 3 because its second-order start
+    steps limit it. A wrong scheme coefficient drops a scheme's order."""
+    g = R.Grid(8, 17, 8, LX, LZ)
+    T = 0.64
+    u0 = random_field(8, 17, 8, LX, LZ, norm=1e-4, decay=0.5, seed=9)
+    sols = []
+    for n in (16, 32, 64, 128):
+        fl = R.make_flags(baseflow="ZeroBase", constraint="PressureGradient", dPdx=0.0, nu=1 / 400,
+                          nonlinearity="SkewSymmetric", dealiasing="DealiasXZ", timestepping=stepping,
+                          initstepping=init, dt=T / n)
+        d = R.OracleDNS(g, fl, u0, g.zeros(1))
+        d.advance(n)
+        sols.append(d.state()[0])
+    diffs = [np.sqrt(R.l2dist2(g, sols[i], sols[i + 1])) for i in range(3)]
+    orders = [np.log2(diffs[i] / diffs[i + 1]) for i in range(2)]
+    print(stepping, init, "differences", diffs, "observed orders", orders, flush=True)
+    assert all(order - 0.15 <= p <= order + 1.2 for p in orders), orders
 
 
 def test_nonlinear_identities():
