@@ -17,7 +17,9 @@ Pins, by reference test:
     parabola is a steady state.
   * The order of every time-stepping scheme (dnsalgo.cpp:102-179, 351-395,
     510-576): under dt halving, successive differences of the solution at a
-    fixed time shrink as dt^p with the scheme's own order p.
+    fixed time shrink as dt^p with the scheme's own order p, for a small field
+    (the implicit part) and for one where the nonlinear term matters (its
+    explicit extrapolation).
   * skew = (convection + divergence)/2 and rotational = convection - grad(|u|^2/2)
     identities for the nonlinear term.
   * DNS::CFL (dnsalgo.cpp:63-67, flowfield.cpp:4035-4068): the laminar Couette
@@ -232,6 +234,31 @@ def test_scheme_order(stepping, init, order):
     orders = [np.log2(diffs[i] / diffs[i + 1]) for i in range(2)]
     print(stepping, init, "differences", diffs, "observed orders", orders, flush=True)
     assert all(order - 0.15 <= p <= order + 1.2 for p in orders), orders
+
+
+@pytest.mark.parametrize("stepping,least", [("SBDF2", 1.85), ("SBDF3", 2.6), ("SBDF4", 2.85), ("CNAB2", 1.85)])
+def test_scheme_order_nonlinear(stepping, least):
+    """The explicit extrapolation of the nonlinear term (the beta weights of
+    dnsalgo.cpp:125-136 and the CNAB2 weights): the same measurement with a
+    field of norm 0.2, where the nonlinear term matters, and dt = T/32 ...
+    T/256. The order from the finest pair is still rising toward the scheme's
+    own (SBDF3 2.74, SBDF4 3.41); SBDF3 with SBDF2's extrapolation weights
+    stays at 1.9."""
+    g = R.Grid(8, 17, 8, LX, LZ)
+    T = 0.64
+    u0 = random_field(8, 17, 8, LX, LZ, norm=0.2, decay=0.5, seed=9)
+    sols = []
+    for n in (32, 64, 128, 256):
+        fl = R.make_flags(baseflow="ZeroBase", constraint="PressureGradient", dPdx=0.0, nu=1 / 400,
+                          nonlinearity="SkewSymmetric", dealiasing="DealiasXZ", timestepping=stepping,
+                          initstepping="SMRK2", dt=T / n)
+        d = R.OracleDNS(g, fl, u0, g.zeros(1))
+        d.advance(n)
+        sols.append(d.state()[0])
+    diffs = [np.sqrt(R.l2dist2(g, sols[i], sols[i + 1])) for i in range(3)]
+    p = np.log2(diffs[1] / diffs[2])
+    print(stepping, "differences", diffs, "observed order", p, flush=True)
+    assert p >= least, p
 
 
 def test_nonlinear_identities():
