@@ -100,6 +100,18 @@ void cpuref_dns_get(void* hp, double* u_spec, double* q_spec) {
     if (q_spec) store(h->fields[1], q_spec);
 }
 
+// The caller's fields between advance() calls (DNS::advance copies them into
+// its history, dnsalgo.cpp:207): as tests/dnsParabolaTest.cpp:177-178 sets u
+// after the DNS was built from zero fields.
+void cpuref_dns_set(void* hp, const double* u_spec, const double* q_spec) {
+    auto* h = static_cast<Handle*>(hp);
+    const bool pu = h->fields[0].padded(), pq = h->fields[1].padded();
+    if (u_spec) load(h->fields[0], u_spec);
+    if (q_spec) load(h->fields[1], q_spec);
+    h->fields[0].setPadded(pu);
+    h->fields[1].setPadded(pq);
+}
+
 double cpuref_dns_time(void* hp) { return static_cast<Handle*>(hp)->dns->time(); }
 double cpuref_dns_dpdx(void* hp) { return static_cast<Handle*>(hp)->dns->dPdx(); }
 double cpuref_dns_ubulk(void* hp) { return static_cast<Handle*>(hp)->dns->Ubulk(); }

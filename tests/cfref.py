@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "libcpuref.so")
+# CFREF_LIB points the tests at another build of the oracle (one with a planted
+# mistake, to show that the pins catch it)
+LIB_PATH = os.environ.get("CFREF_LIB") or os.path.join(ROOT, "oracle", "libcpuref.so")
 
 # enums numbered as channelflow/dnsflags.h:26-38
 BASEFLOW = dict(ZeroBase=0, LinearBase=1, ParabolicBase=2, LaminarBase=3, SuctionBase=4, ArbitraryBase=5)
@@ -66,6 +68,7 @@ def lib():
             "cpuref_dns_advance": ([v, i], i),
             "cpuref_dns_reset_dt": ([v, d], i),
             "cpuref_dns_get": ([v, P, P], None),
+            "cpuref_dns_set": ([v, P, P], None),
             "cpuref_dns_time": ([v], d),
             "cpuref_dns_dpdx": ([v], d),
             "cpuref_dns_ubulk": ([v], d),
@@ -152,6 +155,11 @@ class OracleDNS:
 
     def reset_dt(self, dt):
         check(lib().cpuref_dns_reset_dt(self.h, dt))
+
+    def set_state(self, u, q=None):
+        uf = _as_f64(u).copy()
+        qf = _as_f64(q).copy() if q is not None else None
+        lib().cpuref_dns_set(self.h, ptr(uf), ptr(qf))
 
     def state(self):
         u = self.g.zeros(3)
