@@ -165,10 +165,20 @@ def kernel_bytes(grid, method):
         "k_zpass": nxout * X + nzout * X,
         "k_xfwd": nzout * X + nzout * L,
         "k_yfwd": nzout * L + 3 * L,
-        # SBDF3 right-hand side u_n..u_{n-2}, f_n..f_{n-2} read; the per-mode tau
-        # factors (12 Ny-vectors of doubles per mode = 6 L) read; u (3) and q (1) written
-        "k_tau_solve": (18 + 6 + 4) * L,
+        # SBDF3 right-hand side u_n..u_{n-2}, f_n..f_{n-2} read, u (3) and q (1)
+        # written: 22 L. The per-mode tau factors are shared by every member of a
+        # class and are overhead (tau_factor_bytes), not algorithmic bytes.
+        "k_tau_solve": (18 + 4) * L,
     }
+
+
+def tau_factor_bytes(grid):
+    """Bytes of one tau set (one lambda) of a grid class: 12 Ny-vectors of
+    doubles and 6 scalars per live mode, read by every tau launch of the
+    class whatever its member count."""
+    Nx, Ny, Nz = grid
+    M = (2 * (Nx // 3 - 1) + 1) * (Nz // 3)
+    return 8.0 * (12 * Ny * M + 6 * M)
 
 
 # ---------------------------------------------------------------------------

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -192,23 +193,104 @@ inline std::vector<double> laminar_profile(double nu, int constraint, double dPd
 // ---------------------------------------------------------------------------
 // per-geometry constant tables: FFT plans, y operators
 // ---------------------------------------------------------------------------
+// FFT plan of length N (radices 4, 2, 3) and its twiddles exp(-2 pi i j/N),
+// evaluated in long double; the plan's tw pointer is set by the caller.
+inline FFTPlan fft_plan(int N, std::vector<double>& tw) {
+    FFTPlan plan{};
+    int n = N, np = 0;
+    while (n % 4 == 0) { plan.radix[np++] = 4; n /= 4; }
+    while (n % 2 == 0) { plan.radix[np++] = 2; n /= 2; }
+    while (n % 3 == 0) { plan.radix[np++] = 3; n /= 3; }
+    if (n != 1) throw Error(CFG_ERR_ARG, "grid size " + std::to_string(N) + " is not of the form 2^a 3^b");
+    plan.N = N;
+    plan.npass = np;
+    tw.assign(2 * N, 0.0);
+    for (int j = 0; j < N; ++j) {
+        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * j / N;
+        tw[2 * j] = (double)cosl(ang);
+        tw[2 * j + 1] = (double)sinl(ang);
+    }
+    return plan;
+}
+
+inline Geom make_geom(const cfg_grid& gr, bool dealias) {
+    Geom g{};
+    g.Nx = gr.Nx;
+    g.Ny = gr.Ny;
+    g.Nz = gr.Nz;
+    g.kxd = dealias ? gr.Nx / 3 - 1 : gr.Nx / 2 - 1;
+    g.kzd = dealias ? gr.Nz / 3 - 1 : gr.Nz / 2 - 1;
+    g.nkx = 2 * g.kxd + 1;
+    g.nkz = g.kzd + 1;
+    g.M = g.nkx * g.nkz;
+    g.Lx = gr.Lx;
+    g.Lz = gr.Lz;
+    g.a = gr.a;
+    g.b = gr.b;
+    return g;
+}
+
+// y operators of a grid, row-major Ny x Ny (see YMats), summed in long double
+struct YTables {
+    std::vector<double> T, TD, Ti, DTi;
+    std::vector<long double> tcheb;  // T_k(y_n) = cos(pi n k / (Ny-1)), [n][k]
+};
+
+inline YTables y_tables(int N, double a, double b) {
+    YTables yt;
+    const long double L = (long double)(b - a);
+    std::vector<long double> t(N * N), d(N * N, 0.0L), ti(N * N);
+    for (int y = 0; y < N; ++y)
+        for (int k = 0; k < N; ++k) t[y * N + k] = cos_pi_frac((long)y * k, N - 1);
+    for (int k = 0; k < N; ++k)
+        for (int y = 0; y < N; ++y) {
+            const long double sk = (k == 0 || k == N - 1) ? 0.5L : 1.0L;
+            const long double ey = (y == 0 || y == N - 1) ? 1.0L : 2.0L;
+            ti[k * N + y] = sk * ey * t[y * N + k] / (long double)(N - 1);
+        }
+    // d/dy on coefficients: du[n] = sum_{m>n, m-n odd} (4/L) m u[m], du[0] halved
+    for (int n = 0; n < N; ++n)
+        for (int m = n + 1; m < N; m += 2) d[n * N + m] = (4.0L / L) * m * (n == 0 ? 0.5L : 1.0L);
+    yt.T.resize(N * N);
+    yt.TD.resize(N * N);
+    yt.Ti.resize(N * N);
+    yt.DTi.resize(N * N);
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            long double sd = 0.0L, sdi = 0.0L;
+            for (int k = 0; k < N; ++k) {
+                sd += t[i * N + k] * d[k * N + j];
+                sdi += d[i * N + k] * ti[k * N + j];
+            }
+            yt.T[i * N + j] = (double)t[i * N + j];
+            yt.TD[i * N + j] = (double)sd;
+            yt.Ti[i * N + j] = (double)ti[i * N + j];
+            yt.DTi[i * N + j] = (double)sdi;
+        }
+    yt.tcheb = std::move(t);
+    return yt;
+}
+
+// y values of a coefficient profile and of its derivative (long double sums)
+inline std::vector<double> eval_profile(const Geom& g, const std::vector<long double>& tcheb,
+                                        const std::vector<double>& c, bool deriv) {
+    const int N = g.Ny;
+    std::vector<double> cd = deriv ? host_diff(c, g.b - g.a) : c;
+    std::vector<double> v(N);
+    for (int y = 0; y < N; ++y) {
+        long double s = 0.0L;
+        for (int k = 0; k < N; ++k) s += tcheb[(size_t)y * N + k] * (long double)cd[k];
+        v[y] = (double)s;
+    }
+    return v;
+}
+
 struct FFTTable {
     FFTPlan plan{};
     DBuf tw;
     void build(int N) {
-        int n = N, np = 0;
-        while (n % 4 == 0) { plan.radix[np++] = 4; n /= 4; }
-        while (n % 2 == 0) { plan.radix[np++] = 2; n /= 2; }
-        while (n % 3 == 0) { plan.radix[np++] = 3; n /= 3; }
-        if (n != 1) throw Error(CFG_ERR_ARG, "grid size " + std::to_string(N) + " is not of the form 2^a 3^b");
-        plan.N = N;
-        plan.npass = np;
-        std::vector<double> h(2 * N);
-        for (int j = 0; j < N; ++j) {
-            const long double ang = -2.0L * 3.141592653589793238462643383279502884L * j / N;
-            h[2 * j] = (double)cosl(ang);
-            h[2 * j + 1] = (double)sinl(ang);
-        }
+        std::vector<double> h;
+        plan = fft_plan(N, h);
         tw.alloc(2 * N);
         HIPCHK(hipMemcpy(tw.p, h.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice));
         plan.tw = reinterpret_cast<const double2*>(tw.p);
@@ -226,58 +308,21 @@ struct Geometry {
     YMats ymats() const { return YMats{T.p, TD.p, Ti.p, DTi.p}; }
 
     void build(const cfg_grid& gr, bool dealias_xz) {
-        const bool dealias = dealias_xz;
-        this->dealias = dealias ? 1 : 0;
-        g.Nx = gr.Nx;
-        g.Ny = gr.Ny;
-        g.Nz = gr.Nz;
-        g.kxd = dealias ? gr.Nx / 3 - 1 : gr.Nx / 2 - 1;
-        g.kzd = dealias ? gr.Nz / 3 - 1 : gr.Nz / 2 - 1;
-        g.nkx = 2 * g.kxd + 1;
-        g.nkz = g.kzd + 1;
-        g.M = g.nkx * g.nkz;
-        g.Lx = gr.Lx;
-        g.Lz = gr.Lz;
-        g.a = gr.a;
-        g.b = gr.b;
+        dealias = dealias_xz ? 1 : 0;
+        g = make_geom(gr, dealias_xz);
         fx.build(g.Nx);
         fz.build(g.Nz);
         const int N = g.Ny;
-        const long double L = (long double)(gr.b - gr.a);
-        std::vector<long double> t(N * N), d(N * N, 0.0L), ti(N * N);
-        for (int y = 0; y < N; ++y)
-            for (int k = 0; k < N; ++k) t[y * N + k] = cos_pi_frac((long)y * k, N - 1);
-        tcheb = t;
-        for (int k = 0; k < N; ++k)
-            for (int y = 0; y < N; ++y) {
-                const long double sk = (k == 0 || k == N - 1) ? 0.5L : 1.0L;
-                const long double ey = (y == 0 || y == N - 1) ? 1.0L : 2.0L;
-                ti[k * N + y] = sk * ey * t[y * N + k] / (long double)(N - 1);
-            }
-        // d/dy on coefficients: du[n] = sum_{m>n, m-n odd} (4/L) m u[m], du[0] halved
-        for (int n = 0; n < N; ++n)
-            for (int m = n + 1; m < N; m += 2) d[n * N + m] = (4.0L / L) * m * (n == 0 ? 0.5L : 1.0L);
-        std::vector<double> hT(N * N), hTD(N * N), hTi(N * N), hDTi(N * N);
-        for (int i = 0; i < N; ++i)
-            for (int j = 0; j < N; ++j) {
-                long double sd = 0.0L, sdi = 0.0L;
-                for (int k = 0; k < N; ++k) {
-                    sd += t[i * N + k] * d[k * N + j];
-                    sdi += d[i * N + k] * ti[k * N + j];
-                }
-                hT[i * N + j] = (double)t[i * N + j];
-                hTD[i * N + j] = (double)sd;
-                hTi[i * N + j] = (double)ti[i * N + j];
-                hDTi[i * N + j] = (double)sdi;
-            }
+        YTables yt = y_tables(N, gr.a, gr.b);
+        tcheb = yt.tcheb;
         auto up = [&](DBuf& b, const std::vector<double>& h) {
             b.alloc(h.size());
             HIPCHK(hipMemcpy(b.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
         };
-        up(T, hT);
-        up(TD, hTD);
-        up(Ti, hTi);
-        up(DTi, hDTi);
+        up(T, yt.T);
+        up(TD, yt.TD);
+        up(Ti, yt.Ti);
+        up(DTi, yt.DTi);
         // B of nu u'' - lambda u = f on row n >= 2 (helmholtz.cpp:79-95 with the
         // c_n and beta_n of helmholtz.h:65-66): c_{n-2}/(4n(n-1)) f[n-2]
         // - beta_n/(2(n^2-1)) f[n] + beta_{n+2}/(4n(n+1)) f[n+2], the last
@@ -294,17 +339,8 @@ struct Geometry {
         up(HB, hb);
     }
 
-    // y values of a coefficient profile and of its derivative (long double sums)
     std::vector<double> eval_profile(const std::vector<double>& c, bool deriv) const {
-        const int N = g.Ny;
-        std::vector<double> cd = deriv ? host_diff(c, g.b - g.a) : c;
-        std::vector<double> v(N);
-        for (int y = 0; y < N; ++y) {
-            long double s = 0.0L;
-            for (int k = 0; k < N; ++k) s += tcheb[(size_t)y * N + k] * (long double)cd[k];
-            v[y] = (double)s;
-        }
-        return v;
+        return cfg::eval_profile(g, tcheb, c, deriv);
     }
 };
 
@@ -491,9 +527,20 @@ struct PBuf {
 // history rotation) uploads nothing once warm. One cache serves one stream;
 // when its arena fills, it waits for the stream and starts over.
 // ---------------------------------------------------------------------------
+// Arena bytes per cache: 4 MiB, or CFGPU_TABLE_ARENA_BYTES (at least 4 KiB) when
+// set, which tests use to force recycles.
+inline size_t table_arena_bytes() {
+    const char* e = std::getenv("CFGPU_TABLE_ARENA_BYTES");
+    if (e && *e) {
+        const long v = std::atol(e);
+        if (v >= 4096) return (size_t)v;
+    }
+    return size_t(4) << 20;
+}
+
 class TableCache {
    public:
-    explicit TableCache(size_t bytes = size_t(4) << 20) : cap_(bytes) { arena_.alloc(bytes / sizeof(double)); }
+    explicit TableCache(size_t bytes = table_arena_bytes()) : cap_(bytes) { arena_.alloc(bytes / sizeof(double)); }
     const void* get(const void* data, size_t n, hipStream_t st) {
         std::string key(static_cast<const char*>(data), n);
         auto it = map_.find(key);
@@ -626,6 +673,41 @@ inline std::shared_ptr<TauStore> tau_for(const Geometry& geo, double nu, double 
     return ts;
 }
 
+// Base flow U(y), W(y) of the flags as Chebyshev coefficients
+// (NSE::createCFBaseFlow, nse.cpp:589-632).
+inline void base_from_flags(const Geom& g, const cfg_flags& f, std::vector<double>& U, std::vector<double>& W) {
+    const int Ny = g.Ny;
+    U.assign(Ny, 0.0);
+    W.assign(Ny, 0.0);
+    switch (f.baseflow) {
+        case CFG_ZeroBase: break;
+        case CFG_LinearBase: U[1] = 1; break;
+        case CFG_ParabolicBase: U[0] = 0.5; U[2] = -0.5; break;
+        case CFG_SuctionBase:
+            U = laminar_profile(f.nu, CFG_PressureGradient, 0, f.Ubulk, f.Vsuck, g.a, g.b, -0.5, 0.5, Ny);
+            break;
+        case CFG_LaminarBase:
+            U = laminar_profile(f.nu, f.constraint, f.dPdx, f.Ubulk, f.Vsuck, g.a, g.b, f.ulowerwall, f.uupperwall, Ny);
+            W = laminar_profile(f.nu, f.constraint, f.dPdz, f.Wbulk, f.Vsuck, g.a, g.b, f.wlowerwall, f.wupperwall, Ny);
+            break;
+        default: throw Error(CFG_ERR_ARG, "ArbitraryBase needs Ubase/Wbase");
+    }
+}
+
+// The base flow as the nonlinear kernels read it: [4][Ny] = U, dU/dy, W, dW/dy
+// at the y_n (NLMember::base).
+inline std::vector<double> base_values_host(const Geom& g, const std::vector<long double>& tcheb,
+                                            const std::vector<double>& U, const std::vector<double>& W) {
+    std::vector<double> out;
+    out.reserve(4 * g.Ny);
+    for (const std::vector<double>* v : {&U, &W})
+        for (bool d : {false, true}) {
+            const std::vector<double> e = eval_profile(g, tcheb, *v, d);
+            out.insert(out.end(), e.begin(), e.end());
+        }
+    return out;
+}
+
 // ---------------------------------------------------------------------------
 // Flow constants of one field (nse.cpp:212-289, 589-671): base flow U(y),
 // W(y), the constraint bookkeeping, and their device copies.
@@ -695,27 +777,7 @@ class FlowModel {
     }
 
    private:
-    void create_base(const Geom& g) {  // nse.cpp:589-632
-        const int Ny = g.Ny;
-        Ubase_.assign(Ny, 0.0);
-        Wbase_.assign(Ny, 0.0);
-        switch (flags_.baseflow) {
-            case CFG_ZeroBase: break;
-            case CFG_LinearBase: Ubase_[1] = 1; break;
-            case CFG_ParabolicBase: Ubase_[0] = 0.5; Ubase_[2] = -0.5; break;
-            case CFG_SuctionBase:
-                Ubase_ = laminar_profile(flags_.nu, CFG_PressureGradient, 0, flags_.Ubulk, flags_.Vsuck, g.a, g.b, -0.5,
-                                         0.5, Ny);
-                break;
-            case CFG_LaminarBase:
-                Ubase_ = laminar_profile(flags_.nu, flags_.constraint, flags_.dPdx, flags_.Ubulk, flags_.Vsuck, g.a, g.b,
-                                         flags_.ulowerwall, flags_.uupperwall, Ny);
-                Wbase_ = laminar_profile(flags_.nu, flags_.constraint, flags_.dPdz, flags_.Wbulk, flags_.Vsuck, g.a, g.b,
-                                         flags_.wlowerwall, flags_.wupperwall, Ny);
-                break;
-            default: throw Error(CFG_ERR_ARG, "ArbitraryBase needs Ubase/Wbase");
-        }
-    }
+    void create_base(const Geom& g) { base_from_flags(g, flags_, Ubase_, Wbase_); }
 
     void init_constraint(const Geom& g, const std::vector<double>& up) {  // nse.cpp:634-671
         const int Ny = g.Ny;

@@ -21,6 +21,8 @@
 #include <algorithm>
 
 #include <map>
+#include <stdexcept>
+#include <string>
 
 #include "cfg_kernels.hpp"
 #include "cfg_math.hpp"
@@ -681,7 +683,11 @@ static void allow_lds(K kernel, size_t bytes) {
     static std::map<const void*, size_t> done;
     size_t& d = done[(const void*)kernel];
     if (d >= bytes) return;
-    (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    const hipError_t e =
+        hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess)  // the launch that follows would fail without saying why
+        throw std::runtime_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize, " + std::to_string(bytes) +
+                                 " B): " + hipGetErrorString(e));
     d = bytes;
 }
 

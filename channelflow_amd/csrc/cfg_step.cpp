@@ -473,7 +473,22 @@ std::vector<double> Exec::cfl(const std::vector<NLMember>& fields) {
 // Resolves a tick into launch records: member tables are made resident (a
 // copy only for bytes not seen before) and a nonlinear form's tiling is tuned
 // on its first launch at a member-count bucket. Nothing here is captured.
+//
+// The tables of one tick must belong to one generation of the table arena: a
+// recycle in the middle of a plan (an arena filled by a later table) would
+// leave the earlier records pointing at addresses the later tables overwrite.
+// The plan is therefore redone once in the fresh arena, which holds a whole
+// tick's tables.
 void Exec::plan(const std::vector<TickOps*>& ops) {
+    for (int attempt = 0;; ++attempt) {
+        const unsigned long gen0 = cache.generation();
+        plan_once(ops);
+        if (cache.generation() == gen0) return;
+        if (attempt > 0) throw Error(CFG_ERR_ARG, "the launch tables of one tick do not fit the table arena");
+    }
+}
+
+void Exec::plan_once(const std::vector<TickOps*>& ops) {
     const Geom& g = geo->g;
     recs_.clear();
     auto rec = [&](LaunchRec::Kind k) -> LaunchRec& {

@@ -194,7 +194,8 @@ class Exec {
 
    private:
     void reserve(int n);
-    void plan(const std::vector<TickOps*>& ops);  // fills recs_
+    void plan(const std::vector<TickOps*>& ops);  // fills recs_, every table of one arena generation
+    void plan_once(const std::vector<TickOps*>& ops);
     void issue();                                  // launches recs_ on st
     void drop_graphs();
     int cap_ = 0;

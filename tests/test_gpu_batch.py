@@ -171,3 +171,28 @@ def test_server_results_match_single(gpu):
               flush=True)
         assert np.array_equal(u, us) and np.array_equal(q, qs)
         assert t == pytest.approx(ts, abs=1e-14)
+
+
+def test_table_arena_recycle_mid_plan(gpu, monkeypatch):
+    """A tick whose launch tables fill the table arena part-way through its
+    plan: the plan is redone in the fresh arena, so no record keeps an address
+    that a later table of the same tick overwrote. With a 4 KiB arena every few
+    ticks recycle it, in the middle of a plan of mixed forms (rotational and
+    skew-symmetric members in one class); every member must equal its run
+    under the default 4 MiB arena bit for bit, with graphs on."""
+    def run():
+        b = cfa.Batch()
+        ids = []
+        for i in range(4):
+            fl = dict(COUETTE, nonlinearity="Rotational" if i % 2 else "SkewSymmetric")
+            u0 = random_field(16, 17, 16, LX_C, LZ_C, norm=0.05 + 0.02 * i, decay=0.4, seed=300 + i)
+            ids.append(b.add(fields((16, 17, 16), u0), cfa.DNSFlags(**fl)))
+        b.advance(12)
+        return [b.state(k) for k in ids], b.graph_stats()
+
+    ref, st_ref = run()
+    monkeypatch.setenv("CFGPU_TABLE_ARENA_BYTES", "4096")
+    small, st_small = run()
+    print("graph captures/replays: 4 MiB arena", st_ref, "4 KiB arena", st_small, flush=True)
+    for (ur, qr), (us, qs) in zip(ref, small):
+        assert np.array_equal(ur, us) and np.array_equal(qr, qs)
