@@ -31,7 +31,7 @@ EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_des
             "cfg_kernel_name", "cfg_nonlinear", "cfg_solve", "cfg_pressure",
             "cfg_batch_create", "cfg_batch_destroy", "cfg_batch_add", "cfg_batch_remove", "cfg_batch_advance",
             "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_dpdx",
-            "cfg_batch_cfl", "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats",
+            "cfg_batch_cfl", "cfg_batch_reset_dt", "cfg_batch_advance_members", "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats",
             "cfg_server_create", "cfg_server_destroy", "cfg_server_submit", "cfg_server_step", "cfg_server_result",
             "cfg_server_memory")
 NKERNELS = 9
@@ -102,6 +102,8 @@ def library():
         "cfg_batch_time": ([v, i, P], i),
         "cfg_batch_dpdx": ([v, i, P], i),
         "cfg_batch_cfl": ([v, i, P], i),
+        "cfg_batch_reset_dt": ([v, i, d], i),
+        "cfg_batch_advance_members": ([v, i, C.POINTER(C.c_int), C.POINTER(C.c_int)], i),
         "cfg_batch_profile": ([v, i], i),
         "cfg_batch_kernel_times": ([v, P, C.POINTER(C.c_longlong)], i),
         "cfg_batch_set_graphs": ([v, i], i),
@@ -423,6 +425,18 @@ class Batch:
         v = C.c_double()
         _check(library().cfg_batch_cfl(self._h, int(member), C.byref(v)))
         return v.value
+
+    def reset_dt(self, member, dt):
+        """DNS::reset_dt(dt) of one member (dns.cpp:180-185)."""
+        _check(library().cfg_batch_reset_dt(self._h, int(member), float(dt)))
+
+    def advance_members(self, nsteps):
+        """DNS::advance(fields, n) of each member in {member: n}, together; the
+        other members do not move."""
+        ids = list(nsteps)
+        mem = (C.c_int * max(1, len(ids)))(*ids)
+        cnt = (C.c_int * max(1, len(ids)))(*[int(nsteps[k]) for k in ids])
+        _check(library().cfg_batch_advance_members(self._h, len(ids), mem, cnt))
 
     def profile(self, enable=True):
         _check(library().cfg_batch_profile(self._h, int(bool(enable))))

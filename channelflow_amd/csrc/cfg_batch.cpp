@@ -102,6 +102,20 @@ class Batch {
         sync();
     }
 
+    // DNS::advance(fields, nsteps[k]) of the listed members, together; the
+    // other members stay where they are
+    void advance_members(int count, const int* ids, const int* nsteps) {
+        for (int k = 0; k < count; ++k) {
+            (void)get(ids[k]);
+            if (nsteps[k] < 0) throw Error(CFG_ERR_ARG, "negative step count");
+        }
+        for (auto& kv : members_) kv.second->target = kv.second->it->steps();
+        for (int k = 0; k < count; ++k) get(ids[k]).target += nsteps[k];
+        while (tick()) {
+        }
+        sync();
+    }
+
     void sync() {
         for (auto& kv : classes_) kv.second->ex->sync();
     }
@@ -388,6 +402,22 @@ int cfg_batch_dpdx(cfg_batch* b, int member, double* v) {
     return guarded([&] {
         if (!b || !v) throw Error(CFG_ERR_ARG, "null argument");
         *v = b->b.get(member).it->dPdx();
+    });
+}
+
+int cfg_batch_advance_members(cfg_batch* b, int count, const int* members, const int* nsteps) {
+    return guarded([&] {
+        if (!b || count < 0 || (count > 0 && (!members || !nsteps))) throw Error(CFG_ERR_ARG, "bad argument");
+        b->b.advance_members(count, members, nsteps);
+    });
+}
+
+int cfg_batch_reset_dt(cfg_batch* b, int member, double dt) {  // DNS::reset_dt, dns.cpp:180-185
+    return guarded([&] {
+        if (!b || !(dt > 0)) throw Error(CFG_ERR_ARG, "bad argument");
+        BatchMember& m = b->b.get(member);
+        m.it->reset_dt(dt);
+        m.cls->ex->sync();
     });
 }
 

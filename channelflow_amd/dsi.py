@@ -92,35 +92,41 @@ def fd_step(dx_fields, epsDx):
 
 
 class MultiShooting:
-    """Multiple-shooting residual and Jacobian products of plane channel flow
-    at fixed dt, every integration of a call batched.
+    """Multiple-shooting residual and Jacobian products of plane channel flow,
+    every integration of a call batched.
 
     flags: DNSFlags of the integrations (flags.dt is the requested step; as in
-    cfDSI's f^T, TimeStep::adjust_for_T fits it to T). T: the integration time
-    of each shot. symmetry: None (identity) or a callable applied to the last
-    shot's f^T(u). Tnormalize: divide G by T (cfdsi.cpp:80-81).
+    cfDSI's f^T, TimeStep::adjust_for_T fits it to T). With flags.variabledt
+    each integration adapts its own dt to its CFL number every dT, as f^T does
+    (cfdsi.cpp:733-767): a first step probes the CFL and restarts from u with
+    the adjusted dt, then every dT the CFL at its start decides a reset_dt
+    after it. T: the integration time of each shot. symmetry: None (identity)
+    or a callable applied to the last shot's f^T(u). Tnormalize: divide G by T
+    (cfdsi.cpp:80-81).
     """
 
     def __init__(self, Nx, Ny, Nz, Lx, Lz, flags, T, a=-1.0, b=1.0, symmetry=None, Tnormalize=False):
         from . import DNSFlags  # noqa: F401  (flags is a DNSFlags)
         if T <= 0:
             raise ValueError("MultiShooting: T must be positive")
-        if flags.variabledt:
-            raise ValueError("MultiShooting: fixed dt only (flags.variabledt must be False)")
         self.Nx, self.Ny, self.Nz, self.Lx, self.Lz, self.a, self.b = Nx, Ny, Nz, Lx, Lz, a, b
         ts = TimeStep(flags=flags)
         ts.adjust_for_T(T)
         self.T = float(T)
-        self.nsteps = ts.N() * ts.n()
+        self.timestep = ts
+        self.variable = bool(flags.variabledt)
+        self.nsteps = ts.N() * ts.n()  # steps per integration at fixed dt
         self.flags = flags.copy(dt=ts.dt())
         self.symmetry = symmetry
         self.Tnormalize = bool(Tnormalize)
         self.fcount = 0  # integrations run, as the reference's fcount
+        self.dts = []    # final dt of each integration of the last call
 
     # -- integrations ------------------------------------------------------
     def integrate(self, fields):
         """f^T(u) for every velocity field in `fields` (spectral arrays), all
         advanced together in one Batch."""
+        import copy
         from . import Batch, FlowField
         batch = Batch()
         ids = []
@@ -128,8 +134,26 @@ class MultiShooting:
             fu = FlowField(self.Nx, self.Ny, self.Nz, 3, self.Lx, self.Lz, self.a, self.b, data=u)
             fq = FlowField(self.Nx, self.Ny, self.Nz, 1, self.Lx, self.Lz, self.a, self.b)
             ids.append(batch.add([fu, fq], self.flags))
-        batch.advance(self.nsteps)
         self.fcount += len(fields)
+        if not self.variable:
+            batch.advance(self.nsteps)
+            self.dts = [self.flags.dt] * len(ids)
+            return [batch.state(m)[0] for m in ids]
+        # cfdsi.cpp:733-737: one probe step, dt adjusted to its CFL, restart from u
+        dts = [copy.deepcopy(self.timestep) for _ in ids]
+        batch.advance(1)
+        for m, dt, u in zip(ids, dts, fields):
+            dt.adjust(batch.CFL(m))
+            batch.reset_dt(m, dt.dt())
+            batch.set_state(m, [FlowField(self.Nx, self.Ny, self.Nz, 3, self.Lx, self.Lz, self.a, self.b, data=u)])
+        # cfdsi.cpp:747-767: N intervals dT, each n = dT/dt steps of the member's own dt
+        for _ in range(self.timestep.N()):
+            cfl = {m: batch.CFL(m) for m in ids}
+            batch.advance_members({m: dt.n() for m, dt in zip(ids, dts)})
+            for m, dt in zip(ids, dts):
+                if dt.adjust(cfl[m]):
+                    batch.reset_dt(m, dt.dt())
+        self.dts = [dt.dt() for dt in dts]
         return [batch.state(m)[0] for m in ids]
 
     def _residual(self, shots, finals):

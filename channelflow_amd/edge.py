@@ -11,8 +11,9 @@ one after another until one is not laminar. Each blend is an independent
 integration, so here all of a round's blends advance together in one Batch;
 the first such j, and so the new uL and uH, are the same.
 
-Scope: fixed dt (flags.variabledt False; the reference adapts dt by CFL
-inside CalcAttractor), no energy files or saved fields.
+With flags.variabledt each trajectory adapts its own dt to its CFL number
+after every time unit, as CalcAttractor does (edgetracking.cpp:794-801); no
+energy files or saved fields.
 """
 import numpy as np
 
@@ -81,18 +82,18 @@ class EdgeTracker:
     """Attractor tests and bisection rounds of edge tracking over one grid,
     every integration of a call in one Batch.
 
-    flags: DNSFlags of the integrations (fixed dt); a time unit of the
-    attractor test is dt.n() = round(dT / dt) steps, as TimeStep gives it.
-    tMin, tMax: time units before the first test and at most (tMinAttractor,
+    flags: DNSFlags of the integrations; a time unit of the attractor test is
+    dt.n() = round(dT / dt) steps, as TimeStep gives it, and with
+    flags.variabledt every trajectory's dt follows its CFL number. tMin, tMax:
+    time units before the first test and at most (tMinAttractor,
     tMaxAttractor of the reference).
     """
 
     def __init__(self, Nx, Ny, Nz, Lx, Lz, flags, aflags, tMin=0, tMax=10000, a=-1.0, b=1.0):
         from .timestep import TimeStep
-        if flags.variabledt:
-            raise ValueError("EdgeTracker: fixed dt only (flags.variabledt must be False)")
         self.Nx, self.Ny, self.Nz, self.Lx, self.Lz, self.a, self.b = Nx, Ny, Nz, Lx, Lz, a, b
         ts = TimeStep(flags=flags)
+        self.timestep = ts
         self.nstep = ts.n()
         self.flags = flags.copy(dt=ts.dt())
         self.aflags = aflags
@@ -109,6 +110,7 @@ class EdgeTracker:
         767-812): the initial pressure from PressureSolver, then steps of one
         time unit until the field is laminar or turbulent or tMax is reached.
         Returns [(attractor, time units, final u)]."""
+        import copy
         from . import Batch, PressureSolver
         batch = Batch()
         live = {}
@@ -116,16 +118,19 @@ class EdgeTracker:
             fu, fq = self._fields(u)
             PressureSolver(fu, flags=self.flags).solve(fq, fu)
             mid = batch.add([fu, fq], self.flags)
-            live[mid] = dict(k=k, t=0, tDecay=0, u=np.array(u, dtype=np.complex128), result=NO_ATTRACTOR)
+            live[mid] = dict(k=k, t=0, tDecay=0, u=np.array(u, dtype=np.complex128), result=NO_ATTRACTOR,
+                             dt=copy.deepcopy(self.timestep))
         self.integrations += len(us)
         out = [None] * len(us)
         while live:
             for st in live.values():
                 st["t"] += 1
                 st["lastEcf"] = ecf(st["u"], self.a, self.b)
-            batch.advance(self.nstep)
+            batch.advance_members({mid: st["dt"].n() for mid, st in live.items()})
             for mid in list(live):
                 st = live[mid]
+                if st["dt"].variable() and st["dt"].adjust(batch.CFL(mid)):  # edgetracking.cpp:797-801
+                    batch.reset_dt(mid, st["dt"].dt())
                 st["u"] = batch.state(mid)[0]
                 if st["t"] > self.tMin:
                     st["result"], st["tDecay"] = classify(st["u"], st["tDecay"], st["lastEcf"], self.aflags,

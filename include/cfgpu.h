@@ -150,6 +150,15 @@ int cfg_batch_time(cfg_batch* b, int member, double* t);
 int cfg_batch_dpdx(cfg_batch* b, int member, double* dPdx);
 /* DNS::CFL(u) of one member's current velocity (dns.cpp:282-289). */
 int cfg_batch_cfl(cfg_batch* b, int member, double* cfl);
+/* DNS::reset_dt(dt) of one member (dns.cpp:180-185, dnsalgo.cpp:191-200): its
+ * tau data are rebuilt for dt and its next advance restarts the
+ * initialisation steps, as the reference's callers do each dT
+ * (channelflow/cfdsi.cpp:733-767, programs/edgetracking.cpp:517-520). */
+int cfg_batch_reset_dt(cfg_batch* b, int member, double dt);
+/* DNS::advance(fields, nsteps[k]) of the `count` listed members, together:
+ * members whose dt differ advance their own step counts over one dT. Members
+ * not listed do not move. */
+int cfg_batch_advance_members(cfg_batch* b, int count, const int* members, const int* nsteps);
 /* Kernel timing summed over the batch's streams (as cfg_dns_profile). */
 int cfg_batch_profile(cfg_batch* b, int enable);
 int cfg_batch_kernel_times(cfg_batch* b, double* total_ms, long long* counts);

@@ -86,5 +86,8 @@ def test_multishooting_time_fit():
         ts.adjust_for_T(T)
         assert ms.nsteps == ts.N() * ts.n() and ms.flags.dt == ts.dt()
         assert abs(ms.nsteps * ms.flags.dt - T) <= 1e-12 * T
+    # variable dt: the same fit, and the integrations then adapt dt per dT (cfdsi.cpp:733-767)
+    ms = MultiShooting(16, 17, 16, 1.0, 1.0, cfa.DNSFlags(dt=0.02, variabledt=True), 1.0)
+    assert ms.variable and ms.timestep.N() * ms.timestep.dT() == pytest.approx(1.0, rel=1e-14)
     with pytest.raises(ValueError):
-        MultiShooting(16, 17, 16, 1.0, 1.0, cfa.DNSFlags(dt=0.02, variabledt=True), 1.0)
+        MultiShooting(16, 17, 16, 1.0, 1.0, cfa.DNSFlags(dt=0.02), -1.0)

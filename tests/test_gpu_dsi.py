@@ -107,3 +107,45 @@ def test_residual_and_jacobian_vs_oracle(gpu):
         ej = rel(g, DG[0][i], DGo[i])
         print(f"shot {i}: eps {e:.3e}, Jacobian product rel L2 vs oracle {ej:.2e}", flush=True)
         assert ej <= 1e-6
+
+
+def test_residual_variable_dt_matches_single_and_oracle(gpu):
+    """flags.variabledt: f^T as cfDSI::f computes it (cfdsi.cpp:733-767) -- a
+    probe step whose CFL adjusts dt, a restart from u, then every dT the CFL at
+    its start decides a reset_dt -- for every shot inside one Batch, equal bit
+    for bit to a single DNS driven the same way, and within 1e-10 of the
+    oracle driven the same way."""
+    import copy
+    shots, _ = shots_and_dirs(nshot=3, ndir=0)
+    fl = cfa.DNSFlags(**dict(COUETTE, variabledt=True, dT=0.1, dtmax=0.05))
+    ms = MultiShooting(NX, NY, NZ, LX_C, LZ_C, fl, 1.0)
+    finals = ms.integrate(shots)
+    g = R.Grid(NX, NY, NZ, LX_C, LZ_C)
+    for i, u in enumerate(shots):
+        dt = copy.deepcopy(ms.timestep)
+        d = cfa.DNS([cfa.FlowField(NX, NY, NZ, 3, LX_C, LZ_C, data=u), cfa.FlowField(NX, NY, NZ, 1, LX_C, LZ_C)],
+                    ms.flags)
+        o = R.OracleDNS(g, R.make_flags(**dict(COUETTE, dt=ms.flags.dt)), u, g.zeros(1))
+        d.advance_resident(1)
+        o.advance(1)
+        dt.adjust(d.CFL())
+        d.reset_dt(dt.dt())
+        o.reset_dt(dt.dt())
+        d.set_state([cfa.FlowField(NX, NY, NZ, 3, LX_C, LZ_C, data=u)])
+        o.set_state(u)
+        seq = [dt.dt()]
+        for _ in range(ms.timestep.N()):
+            cfl, cflo = d.CFL(), o.cfl()
+            assert abs(cfl - cflo) <= 1e-12 * cflo
+            d.advance_resident(dt.n())
+            o.advance(dt.n())
+            if dt.adjust(cfl):
+                d.reset_dt(dt.dt())
+                o.reset_dt(dt.dt())
+            seq.append(dt.dt())
+        us = d.state()[0]
+        print(f"shot {i}: dt sequence {seq}", flush=True)
+        assert np.array_equal(finals[i], us)
+        assert ms.dts[i] == dt.dt()
+        assert rel(g, us, o.state()[0]) <= 1e-10
+    assert any(x != COUETTE["dt"] for x in ms.dts)  # dt did adapt

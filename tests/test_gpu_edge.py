@@ -92,3 +92,45 @@ def test_bisection_round_matches_sequential(gpu):
     else:
         assert np.array_equal(L, uL)
     assert d == E.rel_l2_diff(L, H)
+
+
+def calc_attractor_single_vardt(u0, flags, aflags, tMin, tMax):
+    """CalcAttractor with its own DNS and TimeStep (edgetracking.cpp:767-812):
+    after every time unit the CFL number decides a reset_dt."""
+    from channelflow_amd.timestep import TimeStep
+    dt = TimeStep(flags=flags)
+    fu = cfa.FlowField(NX, NY, NZ, 3, LX_C, LZ_C, data=u0)
+    fq = cfa.FlowField(NX, NY, NZ, 1, LX_C, LZ_C)
+    cfa.PressureSolver(fu, flags=flags).solve(fq, fu)
+    dns = cfa.DNS([fu, fq], flags.copy(dt=dt.dt()))
+    u = np.array(u0, dtype=np.complex128)
+    result, t, tDecay, dts = E.NO_ATTRACTOR, 0, 0, []
+    while result == E.NO_ATTRACTOR and t < tMax:
+        t += 1
+        lastEcf = E.ecf(u)
+        dns.advance_resident(dt.n())
+        if dt.adjust(dns.CFL()):
+            dns.reset_dt(dt.dt())
+        dts.append(dt.dt())
+        u = dns.state()[0]
+        if t > tMin:
+            result, tDecay = E.classify(u, tDecay, lastEcf, aflags)
+    return result, t, u, dts
+
+
+def test_attractors_variable_dt_match_single_runs(gpu):
+    """flags.variabledt: each trajectory of the batched call adapts its own dt
+    (its own step count per time unit) and ends bit for bit where its own DNS
+    with its own TimeStep ends."""
+    fl = cfa.DNSFlags(**dict(FLAGS, variabledt=True, dtmax=0.05))
+    tr = E.EdgeTracker(NX, NY, NZ, LX_C, LZ_C, fl, AFLAGS, tMin=1, tMax=12)
+    us = members()
+    got = tr.attractors(us)
+    changed = False
+    for u0, (a, t, u) in zip(us, got):
+        a1, t1, u1, dts = calc_attractor_single_vardt(u0, tr.flags, AFLAGS, 1, 12)
+        print(a, t, a1, t1, "dt sequence", sorted(set(dts)), flush=True)
+        changed = changed or len(set(dts)) > 1 or dts[0] != FLAGS["dt"]
+        assert (a, t) == (a1, t1)
+        assert np.array_equal(u, u1)
+    assert changed  # dt did adapt
