@@ -28,7 +28,7 @@ EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_des
             "cfg_dns_set_state", "cfg_dns_get_state", "cfg_dns_reset_dt", "cfg_dns_reset_time",
             "cfg_dns_reset_gradp", "cfg_dns_reset_bulkv", "cfg_dns_time", "cfg_dns_dpdx", "cfg_dns_ubulk",
             "cfg_dns_cfl", "cfg_dns_base", "cfg_dns_device_state", "cfg_dns_profile", "cfg_dns_kernel_times",
-            "cfg_kernel_name", "cfg_nonlinear", "cfg_solve", "cfg_pressure",
+            "cfg_kernel_name", "cfg_nonlinear", "cfg_nonlinear_noalloc", "cfg_solve", "cfg_pressure",
             "cfg_batch_create", "cfg_batch_destroy", "cfg_batch_add", "cfg_batch_remove", "cfg_batch_advance",
             "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_dpdx",
             "cfg_batch_cfl", "cfg_batch_reset_dt", "cfg_batch_advance_members", "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats",
@@ -89,6 +89,7 @@ def library():
         "cfg_dns_kernel_times": ([v, P, C.POINTER(C.c_longlong)], i),
         "cfg_kernel_name": ([i], C.c_char_p),
         "cfg_nonlinear": ([G, F, P, P], i),
+        "cfg_nonlinear_noalloc": ([G, F, P, P, v], i),
         "cfg_solve": ([G, F, d, P, P, P, P, P], i),
         "cfg_batch_create": ([C.POINTER(v)], i),
         "cfg_batch_destroy": ([v], i),

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcfgpu.so")
 
-SOURCES = ["cfg_kernels.hip", "cfg_step.cpp", "cfg_runtime.cpp", "cfg_batch.cpp"]
+SOURCES = ["cfg_kernels.hip", "cfg_step.cpp", "cfg_runtime.cpp", "cfg_batch.cpp", "cfg_noalloc.cpp"]
 HEADERS = ["cfg_kernels.hpp", "cfg_math.hpp", "cfg_wave.hpp", "cfg_host.hpp", "cfg_step.hpp"]
 
 # Packed FP32 instructions are disabled in every HIP compile of this project.

@@ -115,6 +115,26 @@ int cfg_nonlinear(const cfg_grid* grid, const cfg_flags* flags, const double* u_
 int cfg_solve(const cfg_grid* grid, const cfg_flags* flags, double lambda_t, const double* u0_spec,
               const double* rhs_spec, double* u_spec, double* q_spec, double* dPdx);
 
+/* One NSE::nonlinear evaluation, as cfg_nonlinear, on device-resident fields,
+ * allocating nothing. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc, set-up around HIP graph capture).
+ *   u_dev  velocity, f_dev  nonlinear term out: device pointers to 3-component
+ *          fields in the compact live-mode layout of cfg_dns_device_state
+ *          (complex element (c, ny, ikz, ikx) at ((c*Ny + ny)*nkz + ikz)*nkx + ikx,
+ *          nkx = 2 kxd + 1, nkz = kzd + 1, kxd/kzd = Nx/3-1, Nz/3-1 with
+ *          DealiasXZ, Nx/2-1, Nz/2-1 without).
+ *   stream a hipStream_t, or NULL for the null stream; the call is ordered
+ *          after the caller's work on it and returns once f_dev is written.
+ * Work buffers and tables live in one fixed device array of
+ * CFG_NOALLOC_SCRATCH_BYTES per device (held until the process ends). A grid
+ * that needs more (64x65x64 fits, 96x97x96 does not) is rejected with
+ * CFG_ERR_ARG. Calls on one device share that array: calls that overlap on one
+ * device race. */
+#define CFG_NOALLOC_SCRATCH_BYTES (48u << 20)
+int cfg_nonlinear_noalloc(const cfg_grid* grid, const cfg_flags* flags, const double* u_dev, double* f_dev,
+                          void* stream);
+
 /* PressureSolver(u, Ubase, Wbase, nu, Vsuck, nonlinearity).solve(q, u)
  * (poissonsolver.h:63-97, poissonsolver.cpp:270-431): q with lapl q = -div N(u+U)
  * and dq/dy = nu v'' at the walls, the initial pressure simulateflow.cpp:104-105
