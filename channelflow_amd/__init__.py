@@ -219,19 +219,39 @@ class FlowField:
         return _CGrid(self.Nx, self.Ny, self.Nz, 0, self.Lx, self.Lz, self.a, self.b)
 
     def save(self, filebase, padded=True):
-        """FlowField::save (flowfield.cpp:2603-2640) to channelflow's binary .ff
-        format; the NetCDF/HDF5 formats are not part of this library."""
-        if filebase.endswith((".nc", ".h5")):
-            raise CfgError("FlowField.save: only the binary .ff format is written by this library")
+        """FlowField::save (flowfield.cpp:2598-2640): ".h5" writes channelflow's
+        HDF5 layout (hdf5Save, de-padded physical values), anything else the
+        binary .ff format. NetCDF files are read (from_file), not written."""
+        if filebase.endswith(".nc"):
+            raise CfgError("FlowField.save: NetCDF files are read, not written, by this library; use .h5 or .ff")
+        if filebase.endswith(".h5"):
+            from .h5io import save_h5
+            save_h5(filebase, self.data, self.Nx, self.Ny, self.Nz, self.Lx, self.Lz, self.a, self.b, padded)
+            return
         from .ffio import save_ff
         save_ff(filebase, self.data, self.Nx, self.Ny, self.Nz, self.Lx, self.Lz, self.a, self.b, padded)
 
     @classmethod
     def from_file(cls, filebase):
-        """FlowField(filebase) (flowfield.cpp:215-300) from a spectral .ff file."""
-        from .ffio import load_ff
-        data, m = load_ff(filebase)
-        return cls(m["Nx"], m["Ny"], m["Nz"], m["Nd"], m["Lx"], m["Lz"], m["a"], m["b"], data=data)
+        """FlowField(filebase) (flowfield.cpp:104-300): a NetCDF4 .nc, an HDF5
+        .h5 or a binary .ff file, tried in that order when no suffix is given."""
+        cands = [filebase] if filebase.endswith((".nc", ".h5", ".ff")) else \
+            [filebase + s for s in (".nc", ".h5", ".ff")]
+        for path in cands:
+            if not os.path.exists(path):
+                continue
+            if path.endswith(".nc"):
+                from .h5io import load_nc
+                data, m = load_nc(path)
+                return cls(m["Nx"], m["Ny"], m["Nz"], 3, m["Lx"], m["Lz"], m["a"], m["b"], data=data)
+            if path.endswith(".h5"):
+                from .h5io import load_h5
+                data, m, _ = load_h5(path)
+                return cls(m["Nxpad"], m["Nypad"], m["Nzpad"], m["Nd"], m["Lx"], m["Lz"], m["a"], m["b"], data=data)
+            from .ffio import load_ff
+            data, m = load_ff(path)
+            return cls(m["Nx"], m["Ny"], m["Nz"], m["Nd"], m["Lx"], m["Lz"], m["a"], m["b"], data=data)
+        raise CfgError(f"FlowField({filebase}): no readable .nc, .h5 or .ff file")
 
     def congruent(self, f):
         return (self.Nx, self.Ny, self.Nz, self.Lx, self.Lz, self.a, self.b) == \

@@ -55,5 +55,29 @@ def build(verbose=False):
     return LIB
 
 
+H5_SOURCE = os.path.join(CSRC, "cfg_h5io.cpp")
+H5_LIB = os.path.join(HERE, "libcfgh5.so")
+HDF5_PREFIXES = ("/opt/conda", "/usr", "/usr/local")
+
+
+def build_h5io(verbose=False):
+    """libcfgh5.so: channelflow's .h5/.nc file layouts over the HDF5 C library
+    (host code, g++). Returns the library path, or None when no HDF5
+    installation with headers is found."""
+    for pre in HDF5_PREFIXES:
+        inc, lib = os.path.join(pre, "include"), os.path.join(pre, "lib")
+        if os.path.exists(os.path.join(inc, "hdf5.h")) and os.path.exists(os.path.join(lib, "libhdf5.so")):
+            tmp = H5_LIB + ".tmp"
+            cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I" + inc, H5_SOURCE, "-o", tmp,
+                   "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
+            os.replace(tmp, H5_LIB)
+            return H5_LIB
+    return None
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_h5io(verbose="-v" in sys.argv))
