@@ -127,14 +127,14 @@ def run_multishooting(cfa, T=4.0, nshot=16, ndir=8, seed=2024, plain_sample=9):
             "batched_over_plain": wall_p / wall, "residual_coefficient_norm": gnorm}
 
 
-def member_field(mem):
+def member_field(mem, Lx=LX_C, Lz=LZ_C):
     from channelflow_amd.fields import random_field
     Nx, Ny, Nz = mem["grid"]
-    u = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=mem["norm"], decay=mem["decay"], seed=mem["seed"],
+    u = random_field(Nx, Ny, Nz, Lx, Lz, norm=mem["norm"], decay=mem["decay"], seed=mem["seed"],
                      member=mem["member"])
     if "delta" in mem:
         s, k = mem["delta"]
-        u = u + 1e-7 * random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=1.0, decay=mem["decay"], seed=s, member=k)
+        u = u + 1e-7 * random_field(Nx, Ny, Nz, Lx, Lz, norm=1.0, decay=mem["decay"], seed=s, member=k)
     return u
 
 
@@ -196,6 +196,7 @@ def run_batched(cfa, members, steps, warmup, prof_steps):
     for mem in members:
         batch.add(flow_fields(cfa, mem["grid"], member_field(mem)), cfa.DNSFlags(**mem["flags"]))
     batch.advance(warmup)                 # SMRK2 start + warm-up SBDF3 steps (and graph captures), untimed
+    extra = steady_state(batch, batch.size()[1])
     c0, r0 = batch.graph_stats()
     t0 = time.perf_counter()
     batch.advance(steps)                  # returns after every stream has drained
@@ -223,7 +224,24 @@ def run_batched(cfa, members, steps, warmup, prof_steps):
             raise SystemExit(f"non-finite state of member {mid}")
     pts = sum(np.prod(m["grid"]) for m in members)
     return dict(value=pts * steps / wall, wall=wall, kt=kt, pts=pts, classes=batch.size()[1],
-                graphs={"captured_in_timed_region": c1 - c0, "replayed_in_timed_region": r1 - r0})
+                graphs={"captured_in_timed_region": c1 - c0, "replayed_in_timed_region": r1 - r0,
+                        "ticks_in_timed_region": steps * batch.size()[1], "warmup_steps_added": extra})
+
+
+def steady_state(batch, nclasses, period=6, max_extra=60):
+    """Extends the warm-up until the ensemble's ticks are all graph replays:
+    a block of `period` steps (a multiple of the SBDF3 history rotation) that
+    captures nothing and replays one graph per class and step. Returns the
+    steps added."""
+    extra = 0
+    while extra < max_extra:
+        c0, r0 = batch.graph_stats()
+        batch.advance(period)
+        extra += period
+        c1, r1 = batch.graph_stats()
+        if c1 == c0 and r1 - r0 == period * nclasses:
+            return extra
+    raise SystemExit(f"bench: no graph steady state after {max_extra} extra warm-up steps")
 
 
 def run_plain(cfa, members, steps, warmup):
@@ -356,38 +374,105 @@ def run_single_c2(cfa, steps, warmup):
                 workload="plane Poiseuille Re_bulk=2800, 192x129x192, rotational NL, SBDF3 dt=0.005")
 
 
-def cpu_baseline(members, steps_timed=2, budget_s=25.0, min_s=5.0):
-    """Oracle (oracle/cpu_ref.cpp, one core) on one member of each grid class:
-    untimed construction and SMRK2 start (dns.cpp:157-161), then timed SBDF3
-    steps, as tools/benchmark.cpp:74-146 times channelflow. The ensemble rate is
-    total grid points / sum over members of the class's time per step."""
+def host_info():
+    """CPU model (/proc/cpuinfo, as lscpu prints it) and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(cpu_model=model, logical_cpus=os.cpu_count(), usable_cpus=len(os.sched_getaffinity(0)))
+
+
+def _oracle_member_step_time(args):
+    """Seconds per SBDF3 step of the oracle on one member: construction and the
+    2 SMRK2 start steps untimed (dns.cpp:157-161), then at least `min_steps`
+    steps and `min_s` seconds timed, stopping after `max_s` seconds."""
+    mem, min_steps, min_s, max_s = args
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import cfref as R
+    Lx, Lz = mem.get("box", (LX_C, LZ_C))
+    g = R.Grid(*mem["grid"], Lx, Lz)
+    ref = R.OracleDNS(g, R.make_flags(**mem["flags"]), member_field(mem, Lx, Lz), g.zeros(1))
+    ref.advance(2)
+    t0 = time.perf_counter()
+    n = 0
+    while n < min_steps or (time.perf_counter() - t0) < min_s:
+        ref.advance(1)
+        n += 1
+        if time.perf_counter() - t0 > max_s:
+            break
+    return mem["grid"], (time.perf_counter() - t0) / n, n
+
+
+def cpu_baseline(members, steps_timed=2, budget_s=25.0, min_s=5.0, allcore_s=8.0, max_workers=16):
+    """The oracle (oracle/cpu_ref.cpp, FP64, plain mixed-radix FFT) timed as
+    tools/benchmark.cpp:74-146 times channelflow, in two legs:
+      1 core: one member of each grid class in turn; ensemble time = sum over
+        members of their class's time per step.
+      all cores: one member per core, all at once (members spread over the
+        ensemble so the classes keep its proportions), each timed as above;
+        ensemble time = sum over members of their class's loaded time per step
+        / cores (the members packed perfectly onto the cores).
+    Runs before the GPU is touched, so that its worker processes fork from a
+    process with no device state."""
+    import multiprocessing as mp
+    info = host_info()
     per = {}
     spent = 0.0
     for mem in members:
-        key = mem["grid"]
-        if key in per:
+        if mem["grid"] in per:
             continue
-        g = R.Grid(*mem["grid"], LX_C, LZ_C)
-        ref = R.OracleDNS(g, R.make_flags(**mem["flags"]), member_field(mem), g.zeros(1))
-        ref.advance(2)
         t0 = time.perf_counter()
-        n = 0
-        while n < steps_timed or (time.perf_counter() - t0) < min_s:
-            ref.advance(1)
-            n += 1
-            if time.perf_counter() - t0 > budget_s:
-                break
-        dt = time.perf_counter() - t0
-        spent += dt
-        per[key] = dt / n
+        _, per[mem["grid"]], _ = _oracle_member_step_time((mem, steps_timed, min_s, budget_s))
+        spent += time.perf_counter() - t0
     t_ens = sum(per[m["grid"]] for m in members)
     pts = sum(np.prod(m["grid"]) for m in members)
+    workers = max(1, min(max_workers, info["usable_cpus"]))
+    picks = [members[(k * len(members)) // workers] for k in range(workers)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(workers) as pool:
+        res = pool.map(_oracle_member_step_time, [(m, steps_timed, allcore_s, 3 * allcore_s) for m in picks])
+    wall_all = time.perf_counter() - t0
+    loaded = {}
+    for g, t, _ in res:
+        loaded.setdefault(g, []).append(t)
+    loaded = {g: sum(v) / len(v) for g, v in loaded.items()}
+    t_all = sum(loaded[m["grid"]] for m in members) / workers
+    classes = ", ".join("x".join(map(str, k)) for k in per)
     return dict(value=pts / t_ens, unit="grid-pt*steps/s", cores=1, kind="port",
-                sample="one member per grid class (" + ", ".join("x".join(map(str, k)) for k in per) +
-                       f"): construction and 2 SMRK2 start steps untimed, then >= {steps_timed} SBDF3 steps and >= {min_s:.0f} s timed "
-                       f"({spent:.1f} s total) on 1 core; ensemble time = sum of members' class times")
+                sample=f"one member per grid class ({classes}): construction and 2 SMRK2 start steps untimed, "
+                       f"then >= {steps_timed} SBDF3 steps and >= {min_s:.0f} s timed ({spent:.1f} s total) on 1 core; "
+                       "ensemble time = sum of members' class times",
+                ms_per_step_by_grid={"x".join(map(str, k)): round(v * 1e3, 2) for k, v in per.items()},
+                all_cores={"value": pts / t_all, "cores": workers,
+                           "sample": f"{workers} members at once, one per core ({len(picks)} picks spread over the "
+                                     f"ensemble), each >= {allcore_s:.0f} s of SBDF3 steps after an untimed start "
+                                     f"({wall_all:.1f} s wall); ensemble time = sum of members' loaded class times "
+                                     "/ cores",
+                           "ms_per_step_by_grid_loaded": {"x".join(map(str, k)): round(v * 1e3, 2)
+                                                          for k, v in loaded.items()}},
+                fft="the oracle's own mixed-radix complex FFT (no FFTW in this image)", **info)
+
+
+def cpu_baseline_c2(min_steps=20):
+    """The oracle on the 192x129x192 Poiseuille channel: construction and the 2
+    SMRK2 start steps untimed, then >= min_steps SBDF3 steps timed and scaled
+    linearly to 200 steps."""
+    Nx, Ny, Nz = 192, 129, 192
+    mem = dict(grid=(Nx, Ny, Nz), flags=poiseuille(1.0 / 2800.0, 0.005), norm=0.2, decay=0.5, seed=2024, member=0,
+               box=(4 * np.pi, 2 * np.pi))
+    t0 = time.perf_counter()
+    _, per_step, n = _oracle_member_step_time((mem, min_steps, 0.0, 1e9))
+    total = time.perf_counter() - t0
+    return dict(value=Nx * Ny * Nz / per_step, unit="grid-pt*steps/s", cores=1, kind="port",
+                seconds_per_step=per_step, seconds_for_200_steps_scaled=200 * per_step,
+                sample=f"{n} SBDF3 steps timed after an untimed construction and SMRK2 start ({total:.0f} s in all), "
+                       "scaled linearly to 200 steps", **host_info())
 
 
 def roofline(members, kt, steps):
@@ -418,6 +503,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="c3", choices=["c3", "c4"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-c2", action="store_true",
+                    help="also time the oracle on the 192x129x192 channel (>= 20 steps, about 2 minutes)")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -443,11 +530,20 @@ def main():
               "(ensembles shard by member across GPUs; see the sharding notes of the design)", file=sys.stderr)
         sys.exit(2)
 
+    members = c3_members() if args.workload == "c3" else c4_members()
+    cpu = None
+    if not args.no_cpu and not args.c2_only:  # before the GPU is touched (its workers fork from this process)
+        print("bench: cpu baseline (1 core, then all cores)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(members)
+        if args.cpu_c2:
+            print("bench: cpu baseline of the 192x129x192 channel", file=sys.stderr, flush=True)
+            cpu["c2"] = cpu_baseline_c2()
+        print(f"bench: cpu {json.dumps(cpu)}", file=sys.stderr, flush=True)
+
     import channelflow_amd as cfa
     if args.c2_only:
         print(json.dumps({"single_c2": run_single_c2(cfa, args.steps, args.warmup)}), flush=True)
         return
-    members = c3_members() if args.workload == "c3" else c4_members()
     label = ("C3 ensemble: 64 plane Couette Re=400 integrations (24x 32x33x32, 24x 48x49x48, 16x 64x65x64), "
              "skew-symmetric NL, bulk velocity, SBDF3 dt=0.02" if args.workload == "c3" else
              "C4 multi-shooting: 16 base fields x (1 + 8 perturbed copies), 144 plane Couette Re=400 "
@@ -511,11 +607,12 @@ def main():
     if not args.no_c2:
         print("bench: single 192x129x192", file=sys.stderr, flush=True)
         out["single_c2"] = run_single_c2(cfa, 50, 10)
-    if not args.no_cpu:
-        print("bench: cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(members)
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+        out["gpu_over_cpu"] = {"one_core": b["value"] / cpu["value"], "all_cores": b["value"] / cpu["all_cores"]["value"]}
     if args.copy_calibration:
         copy_calibration()
+        combine_calibration(cfa)
     print(json.dumps(out), flush=True)
 
 
@@ -550,6 +647,32 @@ def copy_calibration(nbytes=COPY_CAL_BYTES, reps=3):
     chk(hip.hipFree(a), "hipFree")
     chk(hip.hipFree(b), "hipFree")
     print(f"copy calibration: {reps} device copies of {nbytes} bytes", file=sys.stderr, flush=True)
+
+
+COMBINE_GRID = (192, 129, 192)
+
+
+def combine_field_doubles(grid):
+    """Doubles in one 3-component field of the compact live-mode layout."""
+    Nx, Ny, Nz = grid
+    return 2 * 3 * Ny * (2 * (Nx // 3 - 1) + 1) * (Nz // 3)
+
+
+def combine_calibration(cfa, steps=2):
+    """A CNRK2 integration on COMBINE_GRID: every Runge-Kutta substep runs one
+    k_combine of two terms over one 3-component field (dnsalgo.cpp:420-446),
+    which reads exactly 2 and writes exactly 1 such field. A counter run checks
+    the copy-based scale of FETCH_SIZE / WRITE_SIZE on it."""
+    from channelflow_amd.fields import random_field
+    Nx, Ny, Nz = COMBINE_GRID
+    Lx, Lz = 4 * np.pi, 2 * np.pi
+    fl = dict(poiseuille(1.0 / 2800.0, 0.005), timestepping="CNRK2", initstepping="CNRK2")
+    u0 = random_field(Nx, Ny, Nz, Lx, Lz, norm=0.2, decay=0.5, seed=2024)
+    dns = cfa.DNS([cfa.FlowField(Nx, Ny, Nz, 3, Lx, Lz, data=u0), cfa.FlowField(Nx, Ny, Nz, 1, Lx, Lz)],
+                  cfa.DNSFlags(**fl))
+    dns.advance_resident(steps)
+    dns.state()
+    print(f"combine calibration: {3 * steps} k_combine launches over {COMBINE_GRID}", file=sys.stderr, flush=True)
 
 
 # device kernels that fill a step slot of another name (wavefront-local FFT passes)
@@ -593,6 +716,12 @@ def traffic_from_counters(fetch_csv, write_csv, out_path, command):
         for n, vs in acc.items():
             per.setdefault(n, {})[key] = 1024.0 * scale[key] * sum(vs) / len(vs)
     out = {n: d.get("fetch", 0.0) + d.get("write", 0.0) for n, d in per.items()}
+    if "k_combine" in per:  # exact bytes: 2 fields read, 1 written per launch (combine_calibration)
+        nf = 8.0 * combine_field_doubles(COMBINE_GRID)
+        cf = per["k_combine"]
+        out["_combine_check"] = {"grid": COMBINE_GRID, "read_counter": cf.get("fetch"), "read_exact": 2 * nf,
+                                 "read_ratio": cf.get("fetch", 0.0) / (2 * nf), "write_counter": cf.get("write"),
+                                 "write_exact": nf, "write_ratio": cf.get("write", 0.0) / nf}
     out["_per_kernel"] = per
     out["_scale"] = scale
     out["_command"] = command

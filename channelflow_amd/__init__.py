@@ -33,12 +33,20 @@ EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_des
             "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_dpdx",
             "cfg_batch_cfl", "cfg_batch_reset_dt", "cfg_batch_advance_members", "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats",
             "cfg_server_create", "cfg_server_destroy", "cfg_server_submit", "cfg_server_step", "cfg_server_result",
-            "cfg_server_memory")
+            "cfg_server_memory", "cfg_tausolver_create", "cfg_tausolver_destroy", "cfg_tausolver_solve",
+            "cfg_tausolver_solve_bulk", "cfg_tausolver_verify", "cfg_tausolver_verify_bulk")
 NKERNELS = 9
 
 
 class CfgError(RuntimeError):
-    pass
+    """A failed library call; `code` is the status (CFG_ERR_*)."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
+
+
+ERR_ARG, ERR_DEVICE, ERR_NONFINITE, ERR_STATE = -1, -2, -3, -4
 
 
 class _CFlags(C.Structure):
@@ -116,6 +124,12 @@ def library():
                              C.POINTER(C.c_int)], i),
         "cfg_server_result": ([v, C.c_longlong, P, P, P], i),
         "cfg_server_memory": ([v, P], i),
+        "cfg_tausolver_create": ([i, i, d, d, d, d, d, d, i, i, C.POINTER(v)], i),
+        "cfg_tausolver_destroy": ([v], i),
+        "cfg_tausolver_solve": ([v, P, P, P, P, P, P, P], i),
+        "cfg_tausolver_solve_bulk": ([v, P, P, P, d, d, P, P, P, P, P, P], i),
+        "cfg_tausolver_verify": ([v, P, P, P, P, P, P, P, P], i),
+        "cfg_tausolver_verify_bulk": ([v, P, P, P, P, d, d, P, P, P, d, d, P], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -127,7 +141,7 @@ def library():
 
 def _check(rc):
     if rc != 0:
-        raise CfgError(f"cfgpu error {rc}: {library().cfg_last_error().decode()}")
+        raise CfgError(f"cfgpu error {rc}: {library().cfg_last_error().decode()}", rc)
 
 
 def _ptr(a):
@@ -373,6 +387,67 @@ class DNS:
         W = np.zeros(self._grid.Ny)
         _check(library().cfg_dns_base(self._h, _ptr(U), _ptr(W)))
         return U, W
+
+
+class TauSolver:
+    """channelflow's per-mode TauSolver (tausolver.h:33-66) on the device:
+    TauSolver(kx, kz, Lx, Lz, a, b, lambda, nu, Ny, tauCorrection=True).
+
+    solve(Rx, Ry, Rz) -> (u, v, w, P) for complex Chebyshev coefficient arrays of
+    length Ny; solve_bulk(Rx, Ry, Rz, umean, wmean) -> (u, v, w, P, dPdx, dPdz)
+    for kx = kz = 0; verify(u, v, w, P, Rx, Ry, Rz) and verify_bulk(...) return
+    the reference's residual sum."""
+
+    def __init__(self, kx, kz, Lx, Lz, a, b, lam, nu, Ny, tauCorrection=True):
+        self.Ny = int(Ny)
+        self.kx, self.kz = int(kx), int(kz)
+        self.lam, self.nu = float(lam), float(nu)
+        h = C.c_void_p()
+        _check(library().cfg_tausolver_create(self.kx, self.kz, float(Lx), float(Lz), float(a), float(b), self.lam,
+                                              self.nu, self.Ny, int(bool(tauCorrection)), C.byref(h)))
+        self._h = h
+
+    def _vec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        if x.shape != (self.Ny,):
+            raise ValueError(f"expected {self.Ny} Chebyshev coefficients, got shape {x.shape}")
+        return x.view(np.float64)
+
+    def _outs(self):
+        return [np.zeros(self.Ny, dtype=np.complex128) for _ in range(4)]
+
+    def solve(self, Rx, Ry, Rz):
+        R = [self._vec(r) for r in (Rx, Ry, Rz)]
+        o = self._outs()
+        _check(library().cfg_tausolver_solve(self._h, *[_ptr(r) for r in R], *[_ptr(x.view(np.float64)) for x in o]))
+        return tuple(o)
+
+    def solve_bulk(self, Rx, Ry, Rz, umean, wmean):
+        R = [self._vec(r) for r in (Rx, Ry, Rz)]
+        o = self._outs()
+        dx, dz = C.c_double(), C.c_double()
+        _check(library().cfg_tausolver_solve_bulk(self._h, *[_ptr(r) for r in R], float(umean), float(wmean),
+                                                  *[_ptr(x.view(np.float64)) for x in o], C.byref(dx), C.byref(dz)))
+        return (*o, dx.value, dz.value)
+
+    def verify(self, u, v, w, P, Rx, Ry, Rz):
+        X = [self._vec(x) for x in (u, v, w, P, Rx, Ry, Rz)]
+        e = C.c_double()
+        _check(library().cfg_tausolver_verify(self._h, *[_ptr(x) for x in X], C.byref(e)))
+        return e.value
+
+    def verify_bulk(self, u, v, w, P, dPdx, dPdz, Rx, Ry, Rz, umean, wmean):
+        U = [self._vec(x) for x in (u, v, w, P)]
+        R = [self._vec(x) for x in (Rx, Ry, Rz)]
+        e = C.c_double()
+        _check(library().cfg_tausolver_verify_bulk(self._h, *[_ptr(x) for x in U], float(dPdx), float(dPdz),
+                                                   *[_ptr(x) for x in R], float(umean), float(wmean), C.byref(e)))
+        return e.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            library().cfg_tausolver_destroy(self._h)
+            self._h = None
 
 
 class Batch:

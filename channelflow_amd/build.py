@@ -12,7 +12,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcfgpu.so")
 
-SOURCES = ["cfg_kernels.hip", "cfg_step.cpp", "cfg_runtime.cpp", "cfg_batch.cpp", "cfg_noalloc.cpp"]
+SOURCES = ["cfg_kernels.hip", "cfg_step.cpp", "cfg_runtime.cpp", "cfg_batch.cpp", "cfg_noalloc.cpp",
+           "cfg_tausolver.cpp"]
 HEADERS = ["cfg_kernels.hpp", "cfg_math.hpp", "cfg_wave.hpp", "cfg_host.hpp", "cfg_step.hpp"]
 
 # Packed FP32 instructions are disabled in every HIP compile of this project.
@@ -78,6 +79,24 @@ def build_h5io(verbose=False):
     return None
 
 
+DRIVER_SOURCE = os.path.join(ROOT, "tests", "cxx", "dns_driver.cpp")
+DRIVER = os.path.join(ROOT, "tests", "cxx", "dns_driver.bin")
+
+
+def build_driver(verbose=False):
+    """tests/cxx/dns_driver.bin: a C++ caller compiled with g++ against
+    include/cfgpu.hpp and libcfgpu.so (the compiled side of the boundary)."""
+    tmp = DRIVER + ".tmp"
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(ROOT, "include"),
+           DRIVER_SOURCE, "-L" + HERE, "-lcfgpu", "-Wl,-rpath,$ORIGIN/../../channelflow_amd", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
     print(build_h5io(verbose="-v" in sys.argv))
+    print(build_driver(verbose="-v" in sys.argv))

@@ -27,6 +27,7 @@ struct BatchMember {
     BatchClass* cls;
     std::unique_ptr<Integrator> it;
     long target = 0;  // steps to reach in the current advance / serving horizon
+    bool finite = true;  // state finite after its last advance
 };
 
 class Batch {
@@ -100,6 +101,33 @@ class Batch {
         while (tick()) {
         }
         sync();
+        if (n > 0) {
+            std::vector<int> ids;
+            for (auto& kv : members_) ids.push_back(kv.first);
+            check_finite(ids);
+        }
+    }
+
+    // the isfinite(L2Norm(u)) test of cfdsi.cpp:769-774 on the listed members'
+    // states, one launch per class; throws naming the members that fail (their
+    // states stay as they are)
+    void check_finite(const std::vector<int>& ids) {
+        std::map<BatchClass*, std::vector<BatchMember*>> by;
+        for (int id : ids) {
+            BatchMember& m = get(id);
+            by[m.cls].push_back(&m);
+        }
+        std::string bad;
+        for (auto& kv : by) {
+            std::vector<const double*> f;
+            for (BatchMember* m : kv.second) f.push_back(m->it->state().u);
+            const std::vector<bool> nf = kv.first->ex->nonfinite(f, kv.first->ex->nfield);
+            for (size_t i = 0; i < nf.size(); ++i) {
+                kv.second[i]->finite = !nf[i];
+                if (nf[i]) bad += (bad.empty() ? "" : ", ") + std::to_string(kv.second[i]->id);
+            }
+        }
+        if (!bad.empty()) throw Error(CFG_ERR_NONFINITE, "velocity not finite after advance: member(s) " + bad);
     }
 
     // DNS::advance(fields, nsteps[k]) of the listed members, together; the
@@ -114,6 +142,10 @@ class Batch {
         while (tick()) {
         }
         sync();
+        std::vector<int> moved;
+        for (int k = 0; k < count; ++k)
+            if (nsteps[k] > 0) moved.push_back(ids[k]);
+        check_finite(moved);
     }
 
     void sync() {
@@ -258,6 +290,10 @@ class Server {
         (void)hipEventDestroy(d.ev);
         (void)hipHostFree(d.host);
         done_.erase(it);
+        // cfdsi.cpp:769-774: a non-finite f^T(u) is an error (the result is still handed out)
+        for (double v : hu)
+            if (!std::isfinite(v))
+                throw Error(CFG_ERR_NONFINITE, "request " + std::to_string(id) + ": velocity not finite at its horizon");
     }
 
     double used() const { return used_; }

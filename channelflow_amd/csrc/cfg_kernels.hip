@@ -757,6 +757,29 @@ void launch_combine(long n, int nterms, const double* coef, const double* const*
 }
 
 // ===========================================================================
+// Finiteness of member states (the isfinite(L2Norm(u)) checks of
+// cfdsi.cpp:707-710, 769-774): flags[m] = 1 if field m holds a NaN or an
+// infinity. Blocks stride over the field; a wave with a non-finite value
+// stores the flag (every such store writes the same 1).
+// ===========================================================================
+__global__ void __launch_bounds__(256) k_nonfinite(const double* const* __restrict__ fields, long n,
+                                                   double* __restrict__ flags) {
+    const double* __restrict__ f = fields[blockIdx.y];
+    bool bad = false;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        bad = bad || !isfinite(f[i]);
+    if (__any(bad) && bad) flags[blockIdx.y] = 1.0;
+}
+
+void launch_nonfinite(int nfields, const double* const* fields, long n, double* flags, hipStream_t st) {
+    if (nfields <= 0) return;
+    const int blk = 256;
+    const long want = (n + blk * 8L - 1) / (blk * 8L);  // about 8 values per thread
+    const unsigned gx = (unsigned)std::max(1L, std::min(want, 1024L));
+    k_nonfinite<<<dim3(gx, (unsigned)nfields), blk, 0, st>>>(fields, n, flags);
+}
+
+// ===========================================================================
 // NSE::linear on live modes (nse.cpp:393-477), one thread per mode
 // ===========================================================================
 __global__ void __launch_bounds__(64) k_linear(Geom g, const LinearMember* __restrict__ tab) {

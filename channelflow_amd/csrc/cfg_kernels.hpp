@@ -191,6 +191,8 @@ void launch_tau_solve(const Geom& g, const double* hb, int n, const TauMember* t
 // ---- elementwise ----------------------------------------------------------
 // out = sum_j coef[j] * in[j], over n doubles (up to 8 terms)
 void launch_combine(long n, int nterms, const double* coef, const double* const* in, double* out, hipStream_t st);
+// flags[m] = 1 (left unchanged otherwise) if fields[m][0..n) holds a NaN or an infinity
+void launch_nonfinite(int nfields, const double* const* fields, long n, double* flags, hipStream_t st);
 void launch_linear(const Geom& g, int n, const LinearMember* tab, hipStream_t st);
 
 // ---- nonlinear term ---------------------------------------------------------

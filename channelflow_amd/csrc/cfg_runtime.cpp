@@ -60,6 +60,10 @@ int cfg_dns_advance(cfg_dns* dns, int nsteps) {
         if (!dns || nsteps < 0) throw Error(CFG_ERR_ARG, "bad argument");
         dns->advance(nsteps);
         dns->ex->sync();
+        // cfdsi.cpp:769-774 stops on a non-finite f^T(u); the state is kept for the caller
+        if (nsteps > 0 && dns->ex->nonfinite({dns->it->state().u}, dns->ex->nfield)[0])
+            throw Error(CFG_ERR_NONFINITE, "velocity not finite after advance (t = " +
+                                               std::to_string(dns->it->time()) + ")");
     });
 }
 

@@ -470,6 +470,22 @@ std::vector<double> Exec::cfl(const std::vector<NLMember>& fields) {
     return out;
 }
 
+std::vector<bool> Exec::nonfinite(const std::vector<const double*>& fields, long n) {
+    const int k = (int)fields.size();
+    std::vector<bool> out(k, false);
+    if (k == 0) return out;
+    if (nfout_.n < (size_t)k) nfout_.alloc(k);
+    HIPCHK(hipMemsetAsync(nfout_.p, 0, k * sizeof(double), st));
+    const double* const* tab = cache.get(fields, st);
+    launch_nonfinite(k, tab, n, nfout_.p, st);
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(k);
+    HIPCHK(hipMemcpyAsync(h.data(), nfout_.p, k * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < k; ++i) out[i] = h[i] != 0.0;
+    return out;
+}
+
 // Resolves a tick into launch records: member tables are made resident (a
 // copy only for bytes not seen before) and a nonlinear form's tiling is tuned
 // on its first launch at a member-count bucket. Nothing here is captured.

@@ -179,6 +179,8 @@ class Exec {
     void run(const std::vector<TickOps*>& ops);
     // CFL factors (flowfield.cpp:4035-4068) of the given fields, synchronously
     std::vector<double> cfl(const std::vector<NLMember>& fields);
+    // for each field (n doubles), whether it holds a NaN or an infinity, synchronously
+    std::vector<bool> nonfinite(const std::vector<const double*>& fields, long n);
     void sync() { HIPCHK(hipStreamSynchronize(st)); }
     void set_graphs(bool on);
     long graph_replays() const { return replays_; }
@@ -199,7 +201,7 @@ class Exec {
     void issue();                                  // launches recs_ on st
     void drop_graphs();
     int cap_ = 0;
-    DBuf yp_, xp_, zo_, fo_, lscr_, cflout_;
+    DBuf yp_, xp_, zo_, fo_, lscr_, cflout_, nfout_;
     std::vector<NLMember> nlt_;
     std::vector<LinearMember> lt_;
     std::vector<TauMember> tt_;

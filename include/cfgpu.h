@@ -67,7 +67,9 @@ int cfg_dns_create(const cfg_grid* grid, const cfg_flags* flags, const double* u
 int cfg_dns_destroy(cfg_dns* dns);
 
 /* DNS::advance(fields, nSteps) -- dns.cpp:135-163. The velocity and pressure
- * stay resident on the device between calls. */
+ * stay resident on the device between calls. Returns CFG_ERR_NONFINITE when
+ * the velocity holds a NaN or an infinity afterwards (the test
+ * channelflow/cfdsi.cpp:769-774 makes on f^T(u)); the state is kept. */
 int cfg_dns_advance(cfg_dns* dns, int nsteps);
 
 /* The caller's `fields` of DNS::advance: set replaces u, q as a caller
@@ -144,6 +146,40 @@ int cfg_nonlinear_noalloc(const cfg_grid* grid, const cfg_flags* flags, const do
 int cfg_pressure(const cfg_grid* grid, const cfg_flags* flags, const double* Ubase, const double* Wbase,
                  const double* u_spec, double* q_spec);
 
+/* ---- TauSolver -----------------------------------------------------------------
+ * channelflow's per-mode Chebyshev-tau Stokes solver (channelflow/tausolver.h:33-66)
+ * for callers that build TauSolvers themselves: it solves
+ *     nu u'' - lambda u - grad P = -R,  div u = 0,  u(a) = u(b) = 0
+ * for the (kx, kz) Fourier mode of (u, v, w, P). Profiles are ComplexChebyCoeff of
+ * length Ny (spectral), passed as Ny interleaved re/im doubles. The tau data and
+ * the solves live on the device (the DNS batches all modes in its own kernels;
+ * this object serves one mode per call). */
+typedef struct cfg_tausolver cfg_tausolver;
+
+/* TauSolver(kx, kz, Lx, Lz, a, b, lambda, nu, Ny, tauCorrection) -- tausolver.cpp:81-176.
+ * Ny odd, >= 5. */
+int cfg_tausolver_create(int kx, int kz, double Lx, double Lz, double a, double b, double lambda, double nu, int Ny,
+                         int taucorrection, cfg_tausolver** out);
+int cfg_tausolver_destroy(cfg_tausolver* s);
+/* solve(u, v, w, P, Rx, Ry, Rz) -- tausolver.cpp:347-402 */
+int cfg_tausolver_solve(cfg_tausolver* s, const double* Rx, const double* Ry, const double* Rz, double* u, double* v,
+                        double* w, double* P);
+/* solve(u, v, w, P, dPdx, dPdz, Rx, Ry, Rz, umean, wmean) -- tausolver.cpp:404-450: the
+ * mean pressure gradients become unknowns and mean(u), mean(w) are imposed; kx = kz = 0
+ * only (CFG_ERR_ARG otherwise). */
+int cfg_tausolver_solve_bulk(cfg_tausolver* s, const double* Rx, const double* Ry, const double* Rz, double umean,
+                             double wmean, double* u, double* v, double* w, double* P, double* dPdx, double* dPdz);
+/* verify(u, v, w, P, Rx, Ry, Rz) -- tausolver.cpp:452-629: the sum of the L2 residuals of
+ * the momentum, pressure and continuity equations, the wall values, and the squared mean
+ * errors (the reference's own sum, including its evaluation of u where it names w at
+ * tausolver.cpp:599-600). */
+int cfg_tausolver_verify(cfg_tausolver* s, const double* u, const double* v, const double* w, const double* P,
+                         const double* Rx, const double* Ry, const double* Rz, double* error);
+/* verify(u, v, w, P, dPdx, dPdz, Rx, Ry, Rz, umean, wmean) -- tausolver.cpp:462-629 */
+int cfg_tausolver_verify_bulk(cfg_tausolver* s, const double* u, const double* v, const double* w, const double* P,
+                              double dPdx, double dPdz, const double* Rx, const double* Ry, const double* Rz,
+                              double umean, double wmean, double* error);
+
 /* ---- Batched integrations ---------------------------------------------------
  * Many independent DNS integrations on one device, of any grids. channelflow's
  * callers run them one after another: edge-tracking bisection members
@@ -161,7 +197,10 @@ int cfg_batch_destroy(cfg_batch* b);
 int cfg_batch_add(cfg_batch* b, const cfg_grid* grid, const cfg_flags* flags, const double* u_spec,
                   const double* q_spec, int* member);
 int cfg_batch_remove(cfg_batch* b, int member);
-/* DNS::advance(fields, nsteps) for every member. */
+/* DNS::advance(fields, nsteps) for every member. Returns CFG_ERR_NONFINITE,
+ * naming the members in cfg_last_error(), when a member's velocity holds a NaN
+ * or an infinity afterwards (channelflow/cfdsi.cpp:769-774); every member has
+ * advanced and keeps its state. */
 int cfg_batch_advance(cfg_batch* b, int nsteps);
 int cfg_batch_size(const cfg_batch* b, int* nmembers, int* nclasses);
 int cfg_batch_get_state(cfg_batch* b, int member, double* u_spec, double* q_spec);
@@ -177,7 +216,7 @@ int cfg_batch_cfl(cfg_batch* b, int member, double* cfl);
 int cfg_batch_reset_dt(cfg_batch* b, int member, double dt);
 /* DNS::advance(fields, nsteps[k]) of the `count` listed members, together:
  * members whose dt differ advance their own step counts over one dT. Members
- * not listed do not move. */
+ * not listed do not move. Non-finite states as cfg_batch_advance. */
 int cfg_batch_advance_members(cfg_batch* b, int count, const int* members, const int* nsteps);
 /* Kernel timing summed over the batch's streams (as cfg_dns_profile). */
 int cfg_batch_profile(cfg_batch* b, int enable);
@@ -206,7 +245,9 @@ int cfg_server_submit(cfg_server* s, const cfg_grid* grid, const cfg_flags* flag
 /* One server step. done receives up to max_done ids of requests whose results
  * reached host memory since the last call (*ndone of them). */
 int cfg_server_step(cfg_server* s, long long* done, int max_done, int* ndone, int* nrunning, int* nqueued);
-/* Final state (reference layout) and time of a completed request; releases it. */
+/* Final state (reference layout) and time of a completed request; releases it.
+ * Returns CFG_ERR_NONFINITE when that velocity holds a NaN or an infinity
+ * (channelflow/cfdsi.cpp:769-774); the outputs are written all the same. */
 int cfg_server_result(cfg_server* s, long long request, double* u_spec, double* q_spec, double* t);
 int cfg_server_memory(const cfg_server* s, double* used_bytes);
 

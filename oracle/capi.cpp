@@ -222,6 +222,30 @@ int cpuref_tausolve(int kx, int kz, double Lx, double Lz, double a, double b, do
     });
 }
 
+// TauSolver's bulk-velocity solve for kx = kz = 0 (tausolver.cpp:404-450): as
+// cpuref_tausolve, plus dP[0] = dPdx and dP[1] = dPdz.
+int cpuref_tausolve_bulk(double Lx, double Lz, double a, double b, double lambda, double nu, int Ny,
+                         int taucorrection, const double* R, double umean, double wmean, double* out, double* dP) {
+    return guard<int>(-1, [&]() -> int {
+        TauSolver ts(0, 0, Lx, Lz, a, b, lambda, nu, Ny, taucorrection != 0);
+        ComplexChebyCoeff Rx(Ny, a, b), Ry(Ny, a, b), Rz(Ny, a, b), u(Ny, a, b), v(Ny, a, b), w(Ny, a, b),
+            P(Ny, a, b);
+        for (int n = 0; n < Ny; ++n) {
+            Rx.set(n, Complex(R[2 * n], R[2 * n + 1]));
+            Ry.set(n, Complex(R[2 * Ny + 2 * n], R[2 * Ny + 2 * n + 1]));
+            Rz.set(n, Complex(R[4 * Ny + 2 * n], R[4 * Ny + 2 * n + 1]));
+        }
+        ts.solve(u, v, w, P, dP[0], dP[1], Rx, Ry, Rz, umean, wmean);
+        ComplexChebyCoeff* o[4] = {&u, &v, &w, &P};
+        for (int c = 0; c < 4; ++c)
+            for (int n = 0; n < Ny; ++n) {
+                out[2 * (c * Ny + n)] = o[c]->re[n];
+                out[2 * (c * Ny + n) + 1] = o[c]->im[n];
+            }
+        return 0;
+    });
+}
+
 // HelmholtzSolver (helmholtz.cpp:18-95): nu u'' - lambda u = f, u(a)=ua, u(b)=ub.
 int cpuref_helmholtz(int Ny, double a, double b, double lambda, double nu, const double* f, double ua, double ub,
                      double* u) {

@@ -79,6 +79,7 @@ def lib():
             "cpuref_linear": ([i, i, i, d, d, d, d, F, P, P, P], i),
             "cpuref_solve": ([i, i, i, d, d, d, d, F, d, P, P, P, P, P], i),
             "cpuref_tausolve": ([i, i, d, d, d, d, d, d, i, i, P, P], i),
+            "cpuref_tausolve_bulk": ([d, d, d, d, d, d, i, i, P, d, d, P, P], i),
             "cpuref_helmholtz": ([i, d, d, d, d, P, d, d, P], i),
             "cpuref_helmholtz_mean": ([i, d, d, d, d, P, d, d, d, P, P], i),
             "cpuref_banded_solve": ([i, P, P, P, P, P], i),
@@ -226,6 +227,16 @@ def tausolve(kx, kz, Lx, Lz, a, b, lam, nu, Ny, taucorr, Rx, Ry, Rz):
     out = np.zeros(4 * Ny, dtype=np.complex128)
     check(lib().cpuref_tausolve(kx, kz, Lx, Lz, a, b, lam, nu, Ny, int(taucorr), ptr(R), ptr(out.view(np.float64))))
     return out.reshape(4, Ny)
+
+
+def tausolve_bulk(Lx, Lz, a, b, lam, nu, Ny, taucorr, Rx, Ry, Rz, umean, wmean):
+    """TauSolver::solve with dPdx, dPdz unknown and mean(u), mean(w) imposed (kx = kz = 0)."""
+    R = np.ascontiguousarray(np.concatenate([Rx, Ry, Rz]).astype(np.complex128)).view(np.float64)
+    out = np.zeros(4 * Ny, dtype=np.complex128)
+    dP = np.zeros(2)
+    check(lib().cpuref_tausolve_bulk(Lx, Lz, a, b, lam, nu, Ny, int(taucorr), ptr(R), umean, wmean,
+                                     ptr(out.view(np.float64)), ptr(dP)))
+    return out.reshape(4, Ny), dP[0], dP[1]
 
 
 def field_from_io(grid, io):

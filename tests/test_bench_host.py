@@ -24,12 +24,13 @@ def test_more_than_one_gpu_refused():
 def test_live_modes_by_enumeration(grid):
     """live() against a count of the (kx, kz) modes kept by 2/3 dealiasing
     (|kx| <= Nx/3 - 1, 0 <= kz <= Nz/3 - 1, flowfield.h:578-584) times Ny;
-    the tau solve's bytes are 28 such component arrays."""
+    the tau solve's algorithmic bytes are 22 such component arrays (u_n..u_{n-2}
+    and f_n..f_{n-2} read, u and q written)."""
     Nx, Ny, Nz = grid
     kept = sum(1 for kx in range(-(Nx // 2) + 1, Nx // 2) for kz in range(Nz // 2)
                if abs(kx) <= Nx // 3 - 1 and kz <= Nz // 3 - 1)
     assert bench.live(grid) == Ny * kept
-    assert bench.kernel_bytes(grid, "SkewSymmetric")["k_tau_solve"] == 28 * 16 * Ny * kept
+    assert bench.kernel_bytes(grid, "SkewSymmetric")["k_tau_solve"] == 22 * 16 * Ny * kept
 
 
 def test_serve_request_stream():

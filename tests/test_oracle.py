@@ -574,3 +574,30 @@ def test_pressure_solver_equations():
                 checked += 1
         print(fl["nonlinearity"], "modes checked", checked)
         assert checked >= 10
+
+
+@pytest.mark.parametrize("N", [17, 33, 49])
+def test_tausolve_bulk_defining_equations(N):
+    """The oracle's bulk-velocity TauSolver::solve (tausolver.cpp:404-450) for
+    kx = kz = 0 against the equations it solves: nu u'' - lambda u - dPdx = -Rx
+    (and w with dPdz) on every Chebyshev row but the top two (the tau rows),
+    u(+-1) = w(+-1) = 0, mean(u) = umean, mean(w) = wmean, v = 0, and
+    P'' = dRy/dy with P(+-1) = 0."""
+    r = np.random.default_rng(N)
+    dec = 0.8 ** np.arange(N)
+    Rx, Ry, Rz = [(r.standard_normal(N) + 1j * r.standard_normal(N)) * dec for _ in range(3)]
+    nu, lam = 1 / 400, 11 / 6 / 0.02
+    out, dPdx, dPdz = R.tausolve_bulk(LX, LZ, -1.0, 1.0, lam, nu, N, True, Rx, Ry, Rz, 0.25, -0.05)
+    u, v, w, P = out
+    d2 = lambda c: _deriv_y(_deriv_y(c))
+    for x, Rr, dP in ((u, Rx, dPdx), (w, Rz, dPdz)):
+        res = nu * d2(x) - lam * x + Rr
+        res[0] -= dP
+        assert np.max(np.abs(res[:N - 2])) < 1e-10 * np.max(np.abs(Rr))
+        assert abs(np.sum(x)) < 1e-12 and abs(np.sum(x * (-1) ** np.arange(N))) < 1e-12
+    mean = lambda c: c[0].real - sum(c[n].real / (n * n - 1) for n in range(2, N, 2))
+    assert abs(mean(u) - 0.25) < 1e-13 and abs(mean(w) + 0.05) < 1e-13
+    assert np.all(v == 0)
+    # P'' = dRy/dy with P(+-1) = 0: P' - Ry is constant (in y) on the non-tau rows
+    assert np.max(np.abs(_deriv_y(_deriv_y(P) - Ry)[:N - 3])) < 1e-10 * np.max(np.abs(Ry))
+    assert abs(np.sum(P)) < 1e-12 and abs(np.sum(P * (-1) ** np.arange(N))) < 1e-12
