@@ -330,7 +330,7 @@ __device__ __forceinline__ CV cview(double* base, const Geom& g, int comp, int m
 // sides per scan); per part the same arithmetic as the one-RHS routines
 // (tausolver.cpp:193-251).
 template <int K, int W>
-__device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, const double* cm, const double* sc, double kappa2,
+__device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, gcdouble cm, gcdouble sc, double kappa2,
                                                   double lambda, double nu, bool taucorr, const WV<K> (&r)[2],
                                                   const WV<K> (&Ry)[2], WV<K> (&P)[2], WV<K> (&v)[2]) {
     const long Ny = c.N;
@@ -386,53 +386,107 @@ __device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, const double* c
     }
 }
 
-// Block of MPB consecutive modes of member blockIdx.y: the right-hand side
-// rows [c][n][m0..m0+MPB) are formed from the member's terms (the multistep /
-// Runge-Kutta combination, dnsalgo.cpp:218-224, 668-673) with full-line loads
-// into LDS as [mode][c][n], each wave solves its mode in registers, and u, v,
-// w, q leave through LDS the same way.
+// The right-hand-side rows [c][n][m0..m0+MPB) of a block: sum_j coef_j in_j
+// (the multistep / Runge-Kutta combination, dnsalgo.cpp:218-224, 668-673),
+// stored to LDS as [mode][c][n]. The term count is a template parameter and
+// UNR elements are loaded per pass, so the NT x UNR loads of a pass are all in
+// flight before the first is summed.
+template <int NT, int MPB>
+__device__ __forceinline__ void tau_stage_rhs(const TauMember& me, const Geom& g, int m0, int nthr, double2* lds) {
+    constexpr int UNR = NT <= 2 ? 8 : (NT <= 4 ? 4 : 2);  // 16 loads in flight at most
+    const __attribute__((address_space(1))) dvec2* in[NT];
+    double coef[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        in[j] = gmem(reinterpret_cast<const dvec2*>(me.in[j]));
+        coef[j] = me.coef[j];
+    }
+    const int Ny = g.Ny;
+    const int total = 3 * Ny * MPB;
+    for (int base = threadIdx.x; base < total; base += UNR * nthr) {
+        dvec2 v[UNR][NT];
+        int slot[UNR];
+#pragma unroll
+        for (int r = 0; r < UNR; ++r) {
+            const int idx = base + r * nthr;
+            const int ww = idx % MPB, rest = idx / MPB, n = rest % Ny, comp = rest / Ny;
+            const int mm = m0 + ww;
+            const bool ok = idx < total && mm < g.M;
+            slot[r] = idx < total ? (ww * 3 + comp) * Ny + n : -1;
+            // element 0 stands in for an element past the block (then zeroed),
+            // so that every load is issued unconditionally
+            const long off = ok ? ((long)comp * Ny + n) * g.M + mm : 0;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                v[r][j] = in[j][off];
+                if (!ok) v[r][j] = dvec2{0.0, 0.0};
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < UNR; ++r) {
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                acc.x += coef[j] * v[r][j].x;
+                acc.y += coef[j] * v[r][j].y;
+            }
+            if (slot[r] >= 0) lds[slot[r]] = acc;
+        }
+    }
+}
+
 // waves per SIMD a tau-solve variant is compiled for: two 16-lane coefficient
 // blocks fit in the 168 registers of three waves
 constexpr int tau_waves(int K, int W) { return (K == 2 && W == 16) ? 3 : 1; }
+constexpr int NXCD = 8;
 
+// Block order of a launch over (mode block, member) pairs: the dispatcher
+// hands consecutive workgroups to the 8 XCDs in turn, so the linear block
+// index is renumbered to put a contiguous range of pairs on each XCD, members
+// fastest. The members of a launch share their tau factors, which one XCD
+// then reads from its own L2 for every member after the first.
+__device__ __forceinline__ void xcd_block(int nmem, int& mode_block, int& member) {
+    const int T = gridDim.x;  // mode blocks x members
+    const int L = blockIdx.x;
+    const int q = T / NXCD, r = T % NXCD, x = L % NXCD;
+    const int p = x * q + (x < r ? x : r) + L / NXCD;
+    mode_block = p / nmem;
+    member = p - mode_block * nmem;
+}
+
+// Block of MPB consecutive modes of one member. Each wave solves its modes in
+// registers from the right-hand sides staged in LDS, and u, v, w, q leave
+// through LDS the same way.
 template <int K, int W, int MPB>
 __global__ void __launch_bounds__(W * MPB) __attribute__((amdgpu_waves_per_eu(tau_waves(K, W))))
-k_tau_solve_wave(Geom g, const double* __restrict__ hb,
-                                                             const TauMember* __restrict__ tab) {
+k_tau_solve_wave(Geom g, const double* __restrict__ hb, int nmem, const TauMember* __restrict__ tab) {
     extern __shared__ double2 lds[];  // [MPB][3][Ny] right-hand sides, then the B rows [3][Ny]
-    const TauMember& me = tab[blockIdx.y];
+    int mblk, mem;
+    xcd_block(nmem, mblk, mem);
+    const TauMember& me = tab[mem];
     const TauSet s = me.set;
     const double nu = me.nu;
     const int taucorr = me.taucorr;
     const SolveConst zc = me.sc;
-    double2* __restrict__ u = reinterpret_cast<double2*>(me.u);
-    double2* __restrict__ q = reinterpret_cast<double2*>(me.q);
-    const int nterms = me.nterms;
+    __attribute__((address_space(1))) dvec2* __restrict__ u = gmem_w(reinterpret_cast<dvec2*>(me.u));
+    __attribute__((address_space(1))) dvec2* __restrict__ q = gmem_w(reinterpret_cast<dvec2*>(me.q));
     const int Ny = g.Ny;
     const int nthr = W * MPB;
-    const int m0 = blockIdx.x * MPB;
+    const int m0 = mblk * MPB;
     const int w = threadIdx.x / W;  // mode slot of this lane's segment
     const int m = m0 + w;
     const bool valid = m < g.M;
     double* hbl = reinterpret_cast<double*>(lds + 3 * Ny * MPB);
     for (int i = threadIdx.x; i < 3 * Ny; i += nthr) hbl[i] = hb[i];
-    for (int idx = threadIdx.x; idx < 3 * Ny * MPB; idx += nthr) {
-        const int ww = idx % MPB, rest = idx / MPB, n = rest % Ny, comp = rest / Ny;
-        const int mm = m0 + ww;
-        double2 acc = make_double2(0.0, 0.0);
-        if (mm < g.M) {
-            const long off = ((long)comp * Ny + n) * g.M + mm;
-#pragma unroll
-            for (int j = 0; j < MAX_TERMS; ++j) {
-                if (j < nterms) {
-                    const double2 v = reinterpret_cast<const double2*>(me.in[j])[off];
-                    const double c = me.coef[j];
-                    acc.x += c * v.x;
-                    acc.y += c * v.y;
-                }
-            }
-        }
-        lds[(ww * 3 + comp) * Ny + n] = acc;
+    switch (me.nterms) {
+        case 1: tau_stage_rhs<1, MPB>(me, g, m0, nthr, lds); break;
+        case 2: tau_stage_rhs<2, MPB>(me, g, m0, nthr, lds); break;
+        case 3: tau_stage_rhs<3, MPB>(me, g, m0, nthr, lds); break;
+        case 4: tau_stage_rhs<4, MPB>(me, g, m0, nthr, lds); break;
+        case 5: tau_stage_rhs<5, MPB>(me, g, m0, nthr, lds); break;
+        case 6: tau_stage_rhs<6, MPB>(me, g, m0, nthr, lds); break;
+        case 7: tau_stage_rhs<7, MPB>(me, g, m0, nthr, lds); break;
+        default: tau_stage_rhs<MAX_TERMS, MPB>(me, g, m0, nthr, lds); break;
     }
     __syncthreads();
     WCtx c;
@@ -452,8 +506,8 @@ k_tau_solve_wave(Geom g, const double* __restrict__ hb,
     const double kappa2 = 4 * pi2 * kk;
     const double lambda = s.lambda_t + (4.0 * pi2 * nu) * kk;
     const double kx2pi = 2 * PI * kx / g.Lx, kz2pi = 2 * PI * kz / g.Lz;
-    const double* cm = s.coef + (long)(valid ? m : 0) * TAU_NA * Ny;
-    const double* sc = s.scal + (long)(valid ? m : 0) * 6;
+    const gcdouble cm = gmem(s.coef) + (long)(valid ? m : 0) * TAU_NA * Ny;
+    const gcdouble sc = gmem(s.scal) + (long)(valid ? m : 0) * 6;
     WV<K> Rxr, Rxi, Ryr, Ryi, Rzr, Rzi;
     if (valid) {
         wload_c<K, W>(lds + (w * 3 + 0) * Ny, c, Rxr, Rxi);
@@ -511,15 +565,15 @@ k_tau_solve_wave(Geom g, const double* __restrict__ hb,
         const int mm = m0 + ww;
         if (mm < g.M) {
             const double2 v = lds[(ww * 3 + slot) * Ny + n];
-            if (slot == 0) q[(long)n * g.M + mm] = v;
-            else u[((long)1 * Ny + n) * g.M + mm] = v;
+            if (slot == 0) q[(long)n * g.M + mm] = dvec2{v.x, v.y};
+            else u[((long)1 * Ny + n) * g.M + mm] = dvec2{v.x, v.y};
         }
     }
     __syncthreads();
     if (valid) {
-        const double* vb = cm + 3L * Ny;
-        const double* vu = cm + 4L * Ny;
-        const double* vd = cm + 5L * Ny;
+        const gcdouble vb = cm + 3L * Ny;
+        const gcdouble vu = cm + 4L * Ny;
+        const gcdouble vd = cm + 5L * Ny;
         // u (slot 0) and w (slot 1) from the velocity Helmholtz operator. Called
         // with explicit register arrays: selecting between them at run time
         // would put them in scratch memory.
@@ -566,7 +620,10 @@ k_tau_solve_wave(Geom g, const double* __restrict__ hb,
     for (int idx = threadIdx.x; idx < 2 * Ny * MPB; idx += nthr) {
         const int ww = idx % MPB, rest = idx / MPB, n = rest % Ny, slot = rest / Ny;
         const int mm = m0 + ww;
-        if (mm < g.M) u[((long)(slot == 0 ? 0 : 2) * Ny + n) * g.M + mm] = lds[(ww * 3 + slot) * Ny + n];
+        if (mm < g.M) {
+            const double2 v = lds[(ww * 3 + slot) * Ny + n];
+            u[((long)(slot == 0 ? 0 : 2) * Ny + n) * g.M + mm] = dvec2{v.x, v.y};
+        }
     }
 }
 
@@ -713,10 +770,11 @@ void launch_tau_solve(const Geom& g, const double* hb, int n, const TauMember* t
     // another's scans
     auto go = [&](auto kern, int w) {
         const int mpb = 256 / w;
-        const dim3 grid((unsigned)((g.M + mpb - 1) / mpb), n);
+        const long nblk = (long)((g.M + mpb - 1) / mpb) * n;
+        if (nblk > 0x7fffffffL) throw std::runtime_error("tau solve: too many blocks in one launch");
         const size_t lds = (size_t)3 * g.Ny * mpb * sizeof(double2) + (size_t)3 * g.Ny * sizeof(double);
         allow_lds(kern, lds);
-        kern<<<grid, 256, lds, st>>>(g, hb, tab);
+        kern<<<dim3((unsigned)nblk), 256, lds, st>>>(g, hb, n, tab);
     };
     // Up to 48 even coefficients: 16-lane segments (four modes per wave, every
     // shuffle a DPP row move) with up to three coefficients per lane. Above:
@@ -782,78 +840,107 @@ void launch_nonfinite(int nfields, const double* const* fields, long n, double* 
 // ===========================================================================
 // NSE::linear on live modes (nse.cpp:393-477), one thread per mode
 // ===========================================================================
-__global__ void __launch_bounds__(64) k_linear(Geom g, const LinearMember* __restrict__ tab) {
+// NSE::linear (nse.cpp:393-477) with one W-lane segment of a wavefront per
+// mode, MPB modes per block, member blockIdx.y: for each velocity component
+// L = nu (u'' - kappa^2 u) - grad q, plus on kx = kz = 0 nu U'', nu W'' and
+// the mean pressure gradient (the bulk-velocity branch takes dPdx from
+// nu (du/dy(b) - du/dy(a)) / Ly, nse.cpp:437-472). Real and imaginary parts
+// share each derivative scan.
+template <int K, int W, int MPB>
+__global__ void __launch_bounds__(W * MPB) k_linear_wave(Geom g, const LinearMember* __restrict__ tab) {
     const LinearMember& me = tab[blockIdx.y];
+    const int m = blockIdx.x * MPB + threadIdx.x / W;
+    if (m >= g.M) return;  // whole segments leave together
     const double nu = me.nu;
-    const LinearConst c = me.c;
-    const double* u = me.u;
-    const double* q = me.q;
-    double* out = me.out;
-    double* scratch = me.scratch;
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= g.M) return;
-    const int ikx = m % g.nkx, ikz = m / g.nkx;
-    const int kx = kx_of(g, ikx), kz = ikz;
-    const long S = g.M;
-    const int N = g.Ny;
-    const double L = g.b - g.a;
-    SV s0{scratch + m, S}, s1{scratch + S * N + m, S}, s2{scratch + 2 * S * N + m, S};
+    const LinearConst lc = me.c;
+    const gcdouble u = gmem(me.u), q = gmem(me.q);
+    __attribute__((address_space(1))) double* out = gmem_w(me.out);
+    const int Ny = g.Ny;
+    WCtx c;
+    c.N = Ny;
+    c.Me = (Ny + 1) / 2;
+    c.Mo = (Ny - 1) / 2;
+    c.Nb = Ny - 1;
+    c.Nc = Ny - 1;
+    c.lane = threadIdx.x % W;
+    c.L = g.b - g.a;
+    c.hb = nullptr;
+    const int ikx = m % g.nkx, kz = m / g.nkx, kx = kx_of(g, ikx);
+    const bool zero = kx == 0 && kz == 0;
     const double kappa2 = 4 * PI * PI * ((kx / g.Lx) * (kx / g.Lx) + (kz / g.Lz) * (kz / g.Lz));
-    const double dx = 2 * PI * kx / g.Lx;
-    const double dz = 2 * PI * kz / g.Lz;
-    CV P = cview(const_cast<double*>(q), g, 0, m);
-    double dPdxAct = 0.0, dPdzAct = 0.0;
+    const long rs = 2L * g.M;  // doubles between consecutive Chebyshev rows
+    auto row0 = [&](int comp) { return 2L * ((long)comp * Ny * g.M + m); };
+    WV<K> Pq[2] = {wload_s<K, W>(q + row0(0), rs, c), wload_s<K, W>(q + row0(0) + 1, rs, c)};
+    double dPd[3] = {0.0, 0.0, 0.0};
+#pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
-        CV U = cview(const_cast<double*>(u), g, comp, m);
-        CV O = cview(out, g, comp, m);
-        for (int part = 0; part < 2; ++part) {
-            SV up = part ? U.im : U.re;
-            for (int n = 0; n < N; ++n) s0[n] = nu * up[n];  // uk = nu u
-            cheb_diff(s0, s1, N, L);
-            if (kx == 0 && kz == 0 && c.bulk && part == 0 && comp != 1) {
-                const double eb = cheb_eval_b(s1, N), ea = cheb_eval_a(s1, N);
-                if (comp == 0) dPdxAct = (eb - ea) / L;
-                else dPdzAct = (eb - ea) / L;
+        WV<K> s0[2] = {wload_s<K, W>(u + row0(comp), rs, c), wload_s<K, W>(u + row0(comp) + 1, rs, c)};
+        for (int p = 0; p < 2; ++p)
+            for (int k = 0; k < K; ++k) {
+                s0[p].e[k] *= nu;  // nu u
+                s0[p].o[k] *= nu;
             }
-            cheb_diff(s1, s2, N, L);  // nu u''
-            SV o = part ? O.im : O.re;
-            for (int n = 0; n < N; ++n) o[n] = s2[n] - kappa2 * s0[n];
+        WV<K> s1[2], s2[2];
+        wdiff_n<K, W, 2>(s0, c, s1);
+        if (zero && lc.bulk && comp != 1) {
+            double eb, ea;
+            wends<K, W>(s1[0], c, eb, ea);
+            dPd[comp] = (eb - ea) / c.L;
         }
+        wdiff_n<K, W, 2>(s1, c, s2);  // nu u''
+        WV<K> o[2];
+        for (int p = 0; p < 2; ++p)
+            for (int k = 0; k < K; ++k) {
+                o[p].e[k] = s2[p].e[k] - kappa2 * s0[p].e[k];
+                o[p].o[k] = s2[p].o[k] - kappa2 * s0[p].o[k];
+            }
         if (comp == 1) {  // - dq/dy
-            for (int part = 0; part < 2; ++part) {
-                SV pp = part ? P.im : P.re;
-                cheb_diff(pp, s1, N, L);
-                SV o = part ? O.im : O.re;
-                for (int n = 0; n < N; ++n) o[n] = o[n] - s1[n];
-            }
-        } else {  // - (i k) q
-            const double d = comp == 0 ? dx : dz;
-            for (int n = 0; n < N; ++n) {
-                // (0, d) * (Pr, Pi) = (-(d Pi), d Pr)
-                O.re[n] = O.re[n] - (0.0 * P.re[n] - d * P.im[n]);
-                O.im[n] = O.im[n] - (0.0 * P.im[n] + d * P.re[n]);
+            WV<K> dq[2];
+            wdiff_n<K, W, 2>(Pq, c, dq);
+            for (int p = 0; p < 2; ++p)
+                for (int k = 0; k < K; ++k) {
+                    o[p].e[k] = o[p].e[k] - dq[p].e[k];
+                    o[p].o[k] = o[p].o[k] - dq[p].o[k];
+                }
+        } else {  // - (i k) q: (0, d) * (Pr, Pi) = (-(d Pi), d Pr)
+            const double d = comp == 0 ? 2 * PI * kx / g.Lx : 2 * PI * kz / g.Lz;
+            for (int k = 0; k < K; ++k) {
+                o[0].e[k] = o[0].e[k] - (0.0 * Pq[0].e[k] - d * Pq[1].e[k]);
+                o[0].o[k] = o[0].o[k] - (0.0 * Pq[0].o[k] - d * Pq[1].o[k]);
+                o[1].e[k] = o[1].e[k] - (0.0 * Pq[1].e[k] + d * Pq[0].e[k]);
+                o[1].o[k] = o[1].o[k] - (0.0 * Pq[1].o[k] + d * Pq[0].o[k]);
             }
         }
-    }
-    if (kx == 0 && kz == 0) {
-        CV O0 = cview(out, g, 0, m), O2 = cview(out, g, 2, m);
-        for (int n = 0; n < N; ++n) O0.re[n] += c.nuUbaseyy[n];
-        for (int n = 0; n < N; ++n) O2.re[n] += c.nuWbaseyy[n];
-        if (!c.bulk) {
-            O0.re[0] -= c.dPdxRef;
-            O2.re[0] -= c.dPdzRef;
-        } else {
-            dPdxAct += c.nuUbase_slope;
-            dPdzAct += c.nuWbase_slope;
-            O0.re[0] -= dPdxAct;
-            O2.re[0] -= dPdzAct;
+        if (zero && comp != 1) {
+            const WV<K> base = wload<K, W>(gmem(comp == 0 ? lc.nuUbaseyy : lc.nuWbaseyy), c);
+            for (int k = 0; k < K; ++k) {
+                o[0].e[k] += base.e[k];
+                o[0].o[k] += base.o[k];
+            }
+            if (c.lane == 0) {
+                if (!lc.bulk) o[0].e[0] -= comp == 0 ? lc.dPdxRef : lc.dPdzRef;
+                else o[0].e[0] -= dPd[comp] + (comp == 0 ? lc.nuUbase_slope : lc.nuWbase_slope);
+            }
         }
+        wstore_s<K, W>(out + row0(comp), rs, o[0], c);
+        wstore_s<K, W>(out + row0(comp) + 1, rs, o[1], c);
     }
 }
 
 void launch_linear(const Geom& g, int n, const LinearMember* tab, hipStream_t st) {
-    const int blk = 64;
-    k_linear<<<dim3((g.M + blk - 1) / blk, n), blk, 0, st>>>(g, tab);
+    const int Me = (g.Ny + 1) / 2;
+    auto go = [&](auto kern, int w) {
+        const int mpb = 256 / w;
+        kern<<<dim3((g.M + mpb - 1) / mpb, n), 256, 0, st>>>(g, tab);
+    };
+    if (Me <= 16) go(k_linear_wave<1, 16, 16>, 16);
+    else if (Me <= 32) go(k_linear_wave<2, 16, 16>, 16);
+    else if (Me <= 48) go(k_linear_wave<3, 16, 16>, 16);
+    else if (Me <= 64) go(k_linear_wave<1, 64, 4>, 64);
+    else if (Me <= 96) go(k_linear_wave<3, 32, 8>, 32);
+    else if (Me <= 128) go(k_linear_wave<2, 64, 4>, 64);
+    else if (Me <= 192) go(k_linear_wave<3, 64, 4>, 64);
+    else throw std::runtime_error("NSE::linear: Ny above 383 is not supported");
 }
 
 // ===========================================================================
@@ -2026,7 +2113,7 @@ __global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((a
     const long m0 = (long)blockIdx.x * YW_MODES;
     const long mcol = m0 + (lr >> 1);
     const bool mok = mcol < M;
-    const double* __restrict__ ucol = me.u + 2 * ((long)c * Ny * M + mcol) + (l & 1);
+    const gcdouble ucol = gmem(me.u) + 2 * ((long)c * Ny * M + mcol) + (l & 1);
     mfma4d ae[NRT], ao[NRT];
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt) ae[rt] = ao[rt] = mfma4d{0.0, 0.0, 0.0, 0.0};
@@ -2100,7 +2187,7 @@ __global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((a
             q[cc] = NMAT == 2 ? make_double2(e[(3 + cc) * ps], e[(3 + cc) * ps + 1]) : make_double2(0.0, 0.0);
         }
         if (m == 0) {  // base flow on the kx = kz = 0 mode (nse.cpp:28-36)
-            const double* __restrict__ base = me.base;
+            const gcdouble base = gmem(me.base);
             p[0].x += base[0 * Ny + y];
             q[0].x += base[1 * Ny + y];
             p[2].x += base[2 * Ny + y];
@@ -2176,7 +2263,7 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
     constexpr bool DB = !(DIVF && NRT >= 6);
     __shared__ double sm[(DB ? 2 : 1) * SA];
     const double* __restrict__ fo = reinterpret_cast<const double*>(fo0 + blockIdx.y * sy);
-    double* __restrict__ f = tab[blockIdx.y].f;
+    __attribute__((address_space(1))) double* __restrict__ f = gmem_w(tab[blockIdx.y].f);
     const int Ny = g.Ny, nh = (Ny + 1) / 2, nm = Ny / 2;
     const bool odd = Ny & 1;
     const long M = g.M;
@@ -2299,7 +2386,7 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
     dlast += __shfl_xor(dlast, 16);
     dlast += __shfl_xor(dlast, 32);
     if (!mok) return;
-    double* __restrict__ fc = f + 2 * ((long)c * Ny * M + mcol) + part;
+    __attribute__((address_space(1))) double* __restrict__ fc = f + 2 * ((long)c * Ny * M + mcol) + part;
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt) {
 #pragma unroll

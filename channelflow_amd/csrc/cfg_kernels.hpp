@@ -154,12 +154,10 @@ struct TauMember {
 };
 
 // NSE::linear of one member: out = nu (u'' - kappa^2 u) - grad q (+ constants).
-// scratch holds 3 real Ny x M arrays.
 struct LinearMember {
     const double* u;
     const double* q;
     double* out;
-    double* scratch;
     double nu;
     LinearConst c;
 };

@@ -42,7 +42,7 @@ void NSECtx::nonlinear(const double* u, double* f, TickOps& ops) {  // nse.cpp:3
 
 void NSECtx::linear(const double* u, const double* q, double* out, TickOps& ops) {  // nse.cpp:393-477
     ops.lin = true;
-    ops.linm = LinearMember{u, q, out, nullptr, flags_.nu, model_->linear_const()};
+    ops.linm = LinearMember{u, q, out, flags_.nu, model_->linear_const()};
 }
 
 void NSECtx::solve(double* u, double* q, std::initializer_list<std::pair<double, const double*>> terms, int s,
@@ -435,7 +435,6 @@ void Exec::reserve(int n) {
     fo_.alloc((size_t)c * nlwork_y_doubles(g));
     xp_.alloc((size_t)c * nlwork_x_doubles(g));
     zo_.alloc((size_t)c * nlwork_x_doubles(g));
-    lscr_.alloc((size_t)c * 3 * g.Ny * g.M);
     cap_ = c;
     ++buf_gen_;
 }
@@ -560,9 +559,7 @@ void Exec::plan_once(const std::vector<TickOps*>& ops) {
     lt_.clear();
     for (TickOps* o : ops)
         if (o->lin) {
-            LinearMember l = o->linm;
-            l.scratch = lscr_.p + (size_t)lt_.size() * 3 * g.Ny * g.M;
-            lt_.push_back(l);
+            lt_.push_back(o->linm);
         }
     if (!lt_.empty()) {
         LaunchRec& r = rec(LaunchRec::LINEAR);

@@ -201,7 +201,7 @@ class Exec {
     void issue();                                  // launches recs_ on st
     void drop_graphs();
     int cap_ = 0;
-    DBuf yp_, xp_, zo_, fo_, lscr_, cflout_, nfout_;
+    DBuf yp_, xp_, zo_, fo_, cflout_, nfout_;
     std::vector<NLMember> nlt_;
     std::vector<LinearMember> lt_;
     std::vector<TauMember> tt_;

@@ -20,6 +20,23 @@
 
 namespace cfg {
 
+// Global-memory view of a pointer read from a launch table (a generic
+// pointer). Loads through it compile to global_load, which only the vector
+// memory counter tracks; through the generic pointer they are flat loads,
+// which the LDS counter tracks too, so every wait for an LDS access would
+// also wait for them.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gmem(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem_w(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+using gcdouble = const __attribute__((address_space(1))) double*;
+// a complex value as a native two-double vector (one 16-byte load or store)
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+
 template <int K>
 struct WV {
     double e[K], o[K];
@@ -107,9 +124,10 @@ __device__ __forceinline__ WV<K> wzero() {
     return v;
 }
 
-// contiguous coefficients c[0..N) (stride 1)
-template <int K, int W>
-__device__ __forceinline__ WV<K> wload(const double* p, const WCtx& c) {
+// contiguous coefficients c[0..N) (stride 1); P is a pointer to double in
+// any address space
+template <int K, int W, class P>
+__device__ __forceinline__ WV<K> wload(P p, const WCtx& c) {
     WV<K> v;
     for (int k = 0; k < K; ++k) {
         const int i = K * c.lane + k;
@@ -119,9 +137,10 @@ __device__ __forceinline__ WV<K> wload(const double* p, const WCtx& c) {
     return v;
 }
 
-// strided coefficients c[n] at p[n * s]
-template <int K, int W>
-__device__ __forceinline__ WV<K> wload_s(const double* p, long s, const WCtx& c) {
+// strided coefficients c[n] at p[n * s] (a pointer to double in any address
+// space)
+template <int K, int W, class P>
+__device__ __forceinline__ WV<K> wload_s(P p, long s, const WCtx& c) {
     WV<K> v;
     for (int k = 0; k < K; ++k) {
         const int i = K * c.lane + k;
@@ -131,8 +150,8 @@ __device__ __forceinline__ WV<K> wload_s(const double* p, long s, const WCtx& c)
     return v;
 }
 
-template <int K, int W>
-__device__ __forceinline__ void wstore_s(double* p, long s, const WV<K>& v, const WCtx& c) {
+template <int K, int W, class P>
+__device__ __forceinline__ void wstore_s(P p, long s, const WV<K>& v, const WCtx& c) {
     for (int k = 0; k < K; ++k) {
         const int i = K * c.lane + k;
         if (i < c.Me) p[(long)(2 * i) * s] = v.e[k];
@@ -463,9 +482,9 @@ __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const doub
 
 // Helmholtz solve of NR profiles sharing one operator with Dirichlet values
 // ua[r] (y=a), ub[r] (y=b); band, up, invd: the operator's UL factors, N each.
-template <int K, int W, int NR>
-__device__ __forceinline__ void whelm_n(const WV<K> (&f)[NR], const double* band, const double* up, const double* invd,
-                                        double lambda, const double (&ua)[NR], const double (&ub)[NR], const WCtx& c,
+template <int K, int W, int NR, class P>
+__device__ __forceinline__ void whelm_n(const WV<K> (&f)[NR], P band, P up, P invd, double lambda,
+                                        const double (&ua)[NR], const double (&ub)[NR], const WCtx& c,
                                         WV<K> (&x)[NR]) {
     const WV<K> B = wload<K, W>(band, c), U = wload<K, W>(up, c), I = wload<K, W>(invd, c);
     const WV<K> HL = wload<K, W>(c.hb, c), HD = wload<K, W>(c.hb + c.N, c), HU = wload<K, W>(c.hb + 2 * c.N, c);
@@ -492,9 +511,9 @@ __device__ __forceinline__ void whelm_n(const WV<K> (&f)[NR], const double* band
 }
 
 // one profile
-template <int K, int W>
-__device__ __forceinline__ WV<K> whelm(const WV<K>& f, const double* band, const double* up, const double* invd,
-                                       double lambda, double ua, double ub, const WCtx& c) {
+template <int K, int W, class P>
+__device__ __forceinline__ WV<K> whelm(const WV<K>& f, P band, P up, P invd, double lambda, double ua, double ub,
+                                       const WCtx& c) {
     const WV<K> f1[1] = {f};
     const double a1[1] = {ua}, b1[1] = {ub};
     WV<K> x1[1];
@@ -515,9 +534,9 @@ __device__ __forceinline__ double wmean(const WV<K>& v, const WCtx& c) {
 
 // Helmholtz solve with unknown constant forcing mu and mean(u) = umean
 // (helmholtz.cpp:158-213); returns mu.
-template <int K, int W>
-__device__ __forceinline__ double whelm_mean(const WV<K>& f, const double* band, const double* up, const double* invd,
-                                             double lambda, double nu, double umean, const WCtx& c, WV<K>& u) {
+template <int K, int W, class P>
+__device__ __forceinline__ double whelm_mean(const WV<K>& f, P band, P up, P invd, double lambda, double nu,
+                                             double umean, const WCtx& c, WV<K>& u) {
     const double uamean = wmean<K, W>(whelm<K, W>(f, band, up, invd, lambda, 0.0, 0.0, c), c);
     WV<K> r = wzero<K, W>();
     if (c.lane == 0) r.e[0] = nu;
