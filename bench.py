@@ -158,7 +158,7 @@ def kernel_bytes(grid, method):
     X = 16.0 * Ny * Nx * (kzd + 1)        # one complex field, x physical, kz spectral
     nyout = {"Rotational": 6, "SkewSymmetric": 6, "Convection": 6, "Divergence": 3}[method]
     nxout = {"Rotational": 6, "SkewSymmetric": 9, "Convection": 9, "Divergence": 3}[method]
-    nzout = {"Rotational": 3, "SkewSymmetric": 9, "Convection": 3, "Divergence": 6}[method]
+    nzout = {"Rotational": 3, "SkewSymmetric": 8, "Convection": 3, "Divergence": 6}[method]
     return {
         "k_yinv": (3 + nyout) * L,
         "k_xinv": nyout * L + nxout * X,
@@ -196,12 +196,24 @@ def run_batched(cfa, members, steps, warmup, prof_steps):
     for mem in members:
         batch.add(flow_fields(cfa, mem["grid"], member_field(mem)), cfa.DNSFlags(**mem["flags"]))
     batch.advance(warmup)                 # SMRK2 start + warm-up SBDF3 steps (and graph captures), untimed
-    extra = steady_state(batch, batch.size()[1])
+    extra = steady_state(batch, 1)
     c0, r0 = batch.graph_stats()
+    k0, l0, h0 = batch.tick_stats()
     t0 = time.perf_counter()
     batch.advance(steps)                  # returns after every stream has drained
     wall = time.perf_counter() - t0
     c1, r1 = batch.graph_stats()
+    k1, l1, h1 = batch.tick_stats()
+    # the same steps with one graph per grid class and tick, the classes on
+    # their own streams
+    batch.set_graphs(cfa.Batch.GRAPHS_PER_CLASS)
+    steady_state(batch, batch.size()[1])
+    k2, l2, h2 = batch.tick_stats()
+    t2 = time.perf_counter()
+    batch.advance(steps)
+    wall_w = time.perf_counter() - t2
+    k3, l3, h3 = batch.tick_stats()
+    batch.set_graphs(True)
     kt = {}
     for g in sorted({m["grid"] for m in members}):
         cls = cfa.Batch()
@@ -224,22 +236,32 @@ def run_batched(cfa, members, steps, warmup, prof_steps):
             raise SystemExit(f"non-finite state of member {mid}")
     pts = sum(np.prod(m["grid"]) for m in members)
     return dict(value=pts * steps / wall, wall=wall, kt=kt, pts=pts, classes=batch.size()[1],
-                graphs={"captured_in_timed_region": c1 - c0, "replayed_in_timed_region": r1 - r0,
-                        "ticks_in_timed_region": steps * batch.size()[1], "warmup_steps_added": extra})
+                graphs={"mode": "one graph per tick for the whole batch (grid classes as parallel branches)",
+                        "captured_in_timed_region": c1 - c0, "replayed_in_timed_region": r1 - r0,
+                        "ticks_in_timed_region": k1 - k0, "graph_launches_per_step": (l1 - l0) / steps,
+                        "host_us_per_tick": (h1 - h0) / max(1, k1 - k0), "warmup_steps_added": extra,
+                        "per_class": {"mode": "one graph per grid class and tick, the classes on their own "
+                                              "streams; same members and steps, after the timed region",
+                                      "value": pts * steps / wall_w, "ms_per_step": wall_w / steps * 1e3,
+                                      "graph_launches_per_step": (l3 - l2) / steps,
+                                      "host_us_per_tick": (h3 - h2) / max(1, k3 - k2)}})
 
 
-def steady_state(batch, nclasses, period=6, max_extra=60):
+def steady_state(batch, per_tick, period=6, max_extra=60):
     """Extends the warm-up until the ensemble's ticks are all graph replays:
     a block of `period` steps (a multiple of the SBDF3 history rotation) that
-    captures nothing and replays one graph per class and step. Returns the
-    steps added."""
+    captures nothing and replays `per_tick` graphs per tick (the grid classes
+    with one graph per class, 1 with one graph for the whole batch). Returns
+    the steps added."""
     extra = 0
     while extra < max_extra:
         c0, r0 = batch.graph_stats()
+        k0 = batch.tick_stats()[0]
         batch.advance(period)
         extra += period
         c1, r1 = batch.graph_stats()
-        if c1 == c0 and r1 - r0 == period * nclasses:
+        k1 = batch.tick_stats()[0]
+        if c1 == c0 and k1 - k0 == period and r1 - r0 == per_tick * period:
             return extra
     raise SystemExit(f"bench: no graph steady state after {max_extra} extra warm-up steps")
 

@@ -31,7 +31,7 @@ EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_des
             "cfg_kernel_name", "cfg_nonlinear", "cfg_nonlinear_noalloc", "cfg_solve", "cfg_pressure",
             "cfg_batch_create", "cfg_batch_destroy", "cfg_batch_add", "cfg_batch_remove", "cfg_batch_advance",
             "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_dpdx",
-            "cfg_batch_cfl", "cfg_batch_reset_dt", "cfg_batch_advance_members", "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats",
+            "cfg_batch_cfl", "cfg_batch_reset_dt", "cfg_batch_advance_members", "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats", "cfg_batch_tick_stats",
             "cfg_server_create", "cfg_server_destroy", "cfg_server_submit", "cfg_server_step", "cfg_server_result",
             "cfg_server_memory", "cfg_tausolver_create", "cfg_tausolver_destroy", "cfg_tausolver_solve",
             "cfg_tausolver_solve_bulk", "cfg_tausolver_verify", "cfg_tausolver_verify_bulk")
@@ -117,6 +117,7 @@ def library():
         "cfg_batch_kernel_times": ([v, P, C.POINTER(C.c_longlong)], i),
         "cfg_batch_set_graphs": ([v, i], i),
         "cfg_batch_graph_stats": ([v, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)], i),
+        "cfg_batch_tick_stats": ([v, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.POINTER(C.c_double)], i),
         "cfg_server_create": ([d, C.POINTER(v)], i),
         "cfg_server_destroy": ([v], i),
         "cfg_server_submit": ([v, G, F, P, P, i, C.POINTER(C.c_longlong)], i),
@@ -543,15 +544,26 @@ class Batch:
         _check(library().cfg_batch_kernel_times(self._h, _ptr(tot), cnt))
         return {library().cfg_kernel_name(i).decode(): (float(tot[i]), int(cnt[i])) for i in range(NKERNELS)}
 
-    def set_graphs(self, enable=True):
-        """hipGraph capture and replay of repeating ticks (on by default)."""
-        _check(library().cfg_batch_set_graphs(self._h, int(bool(enable))))
+    GRAPHS_OFF, GRAPHS_PER_CLASS, GRAPHS_WHOLE = 0, 1, 2
+
+    def set_graphs(self, mode=True):
+        """hipGraph capture and replay of repeating ticks: False / 0 eager
+        launches, 1 one graph per grid class and tick, True / 2 one graph per
+        tick for the whole batch (the default)."""
+        m = self.GRAPHS_WHOLE if mode is True else (self.GRAPHS_OFF if mode is False else int(mode))
+        _check(library().cfg_batch_set_graphs(self._h, m))
 
     def graph_stats(self):
-        """(captures, replays) summed over the grid classes."""
+        """(captures, replays), whole-batch and per-class graphs together."""
         c, r = C.c_longlong(), C.c_longlong()
         _check(library().cfg_batch_graph_stats(self._h, C.byref(c), C.byref(r)))
         return c.value, r.value
+
+    def tick_stats(self):
+        """(ticks, graph launches, host microseconds spent in the ticks)."""
+        t, g, h = C.c_longlong(), C.c_longlong(), C.c_double()
+        _check(library().cfg_batch_tick_stats(self._h, C.byref(t), C.byref(g), C.byref(h)))
+        return t.value, g.value, h.value
 
 
 class PressureSolver:

@@ -20,6 +20,12 @@ namespace cfg {
 struct BatchClass {
     std::unique_ptr<Exec> ex;
     std::vector<int> members;  // ids, in insertion order
+    std::deque<TickOps> ops;   // this tick's operations of the members below their target
+    std::vector<TickOps*> ptrs;
+    hipEvent_t done = nullptr;  // marks the class stream's work for a whole-batch launch
+    ~BatchClass() {
+        if (done) (void)hipEventDestroy(done);
+    }
 };
 
 struct BatchMember {
@@ -30,12 +36,31 @@ struct BatchMember {
     bool finite = true;  // state finite after its last advance
 };
 
+// Graph modes of a batch: none (eager launches), one graph per grid class and
+// tick (each class on its own stream), or one graph per tick for the whole
+// batch (the default), its classes captured as parallel branches forked from
+// the batch's stream: one launch per tick, and the branches overlap on the
+// device as the class streams do. A server's membership changes every few
+// ticks, and a whole-batch key with it, so the server keeps one graph per
+// class.
+enum GraphMode { GRAPHS_OFF = 0, GRAPHS_PER_CLASS = 1, GRAPHS_WHOLE = 2 };
+
 class Batch {
    public:
+    Batch() {
+        HIPCHK(hipStreamCreate(&bst_));
+        HIPCHK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&after_, hipEventDisableTiming));
+    }
     ~Batch() {
         for (auto& kv : classes_) kv.second->ex->sync();
+        (void)hipStreamSynchronize(bst_);
+        drop_whole();
         members_.clear();  // integrators before their Exec
         classes_.clear();
+        (void)hipEventDestroy(fork_);
+        (void)hipEventDestroy(after_);
+        (void)hipStreamDestroy(bst_);
     }
 
     int add(const cfg_grid& grid, const cfg_flags& flags, const double* u, const double* q) {
@@ -46,7 +71,8 @@ class Batch {
         if (cit == classes_.end()) {
             auto c = std::make_unique<BatchClass>();
             c->ex = std::make_unique<Exec>(grid, flags.dealiasing == CFG_DealiasXZ);
-            c->ex->set_graphs(graphs_);
+            c->ex->set_graphs(mode_ == GRAPHS_PER_CLASS);
+            HIPCHK(hipEventCreateWithFlags(&c->done, hipEventDisableTiming));
             cit = classes_.emplace(k, std::move(c)).first;
         }
         BatchClass* cls = cit->second.get();
@@ -76,26 +102,35 @@ class Batch {
 
     // One tick of every member below its target. Returns false when none was.
     bool tick() {
-        bool any = false;
+        const auto t0 = std::chrono::steady_clock::now();
+        active_.clear();
+        bool timing = false;
         for (auto& kv : classes_) {
             BatchClass& c = *kv.second;
-            ops_.clear();
-            ptrs_.clear();
+            c.ops.clear();
+            c.ptrs.clear();
             for (int id : c.members) {
                 BatchMember& m = *members_.at(id);
                 if (m.it->steps() < m.target) {
-                    ops_.emplace_back();
-                    m.it->next(ops_.back());
+                    c.ops.emplace_back();
+                    m.it->next(c.ops.back());
                 }
             }
-            if (ops_.empty()) continue;
-            for (auto& o : ops_) ptrs_.push_back(&o);
-            c.ex->run(ptrs_);
-            any = true;
+            if (c.ops.empty()) continue;
+            for (auto& o : c.ops) c.ptrs.push_back(&o);
+            active_.push_back(&c);
+            timing = timing || c.ex->kt.on;
         }
-        return any;
+        if (active_.empty()) return false;
+        ++ticks_;
+        if (mode_ != GRAPHS_WHOLE || timing) {  // eager, or each class's own graphs
+            for (BatchClass* c : active_) c->ex->run(c->ptrs);
+        } else {
+            whole_tick();
+        }
+        host_us_ += std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+        return true;
     }
-
     void advance(int n) {
         for (auto& kv : members_) kv.second->target = kv.second->it->steps() + n;
         while (tick()) {
@@ -149,6 +184,7 @@ class Batch {
     }
 
     void sync() {
+        HIPCHK(hipStreamSynchronize(bst_));
         for (auto& kv : classes_) kv.second->ex->sync();
     }
 
@@ -174,13 +210,17 @@ class Batch {
     }
     int nclasses() const { return (int)classes_.size(); }
 
-    // hipGraph capture and replay of each class's repeating ticks (on by default)
-    void set_graphs(bool on) {
-        graphs_ = on;
-        for (auto& kv : classes_) kv.second->ex->set_graphs(on);
+    // hipGraph capture and replay of repeating ticks (GraphMode; whole-batch
+    // graphs by default)
+    void set_graphs(int mode) {
+        if (mode < GRAPHS_OFF || mode > GRAPHS_WHOLE) throw Error(CFG_ERR_ARG, "graph mode must be 0, 1 or 2");
+        sync();
+        drop_whole();
+        mode_ = (GraphMode)mode;
+        for (auto& kv : classes_) kv.second->ex->set_graphs(mode_ == GRAPHS_PER_CLASS);
     }
     void graph_stats(long long* captures, long long* replays) const {
-        long long c = 0, r = 0;
+        long long c = captures_, r = replays_;
         for (auto& kv : classes_) {
             c += kv.second->ex->graph_captures();
             r += kv.second->ex->graph_replays();
@@ -188,13 +228,104 @@ class Batch {
         if (captures) *captures = c;
         if (replays) *replays = r;
     }
+    // ticks so far, graph launches among them (replays and launches right
+    // after a capture), and host microseconds spent planning and launching
+    void tick_stats(long long* ticks, long long* graph_launches, double* host_us) const {
+        long long gl = replays_ + captures_;
+        for (auto& kv : classes_) gl += kv.second->ex->graph_replays() + kv.second->ex->graph_captures();
+        if (ticks) *ticks = ticks_;
+        if (graph_launches) *graph_launches = gl;
+        if (host_us) *host_us = host_us_;
+    }
 
    private:
-    bool graphs_ = true;
+    // One tick of the active classes as one graph on bst_. The key is every
+    // active class's identity, generation and launch records; a key seen for
+    // the second time is captured, with each class's launches on its own
+    // stream forked from bst_ and joined back, so the classes stay parallel
+    // branches of the graph. Class streams and bst_ are ordered with events
+    // around every whole-batch launch, so work a class stream takes outside
+    // these graphs (uploads, copies, checks, frees) stays in order.
+    void whole_tick() {
+        std::string key;
+        for (BatchClass* c : active_) {
+            c->ex->prepare(c->ptrs);
+            const unsigned long gen = c->ex->generation();
+            key.append(reinterpret_cast<const char*>(&c), sizeof c);
+            key.append(reinterpret_cast<const char*>(&gen), sizeof gen);
+            c->ex->append_key(key);
+        }
+        auto it = whole_.find(key);
+        if (it != whole_.end()) {
+            launch_whole(it->second);
+            ++replays_;
+            return;
+        }
+        if (!seen_.count(key)) {  // first occurrence: eager, each class on its stream
+            if (seen_.size() + whole_.size() >= kMaxWholeKeys) {  // membership churn: start over
+                sync();
+                drop_whole();
+            }
+            seen_.insert(key);
+            for (BatchClass* c : active_) c->ex->issue();
+            return;
+        }
+        join_classes();  // earlier class-stream work before the capture's fork
+        hipGraph_t gr = nullptr;
+        HIPCHK(hipStreamBeginCapture(bst_, hipStreamCaptureModeThreadLocal));
+        try {
+            HIPCHK(hipEventRecord(fork_, bst_));
+            for (BatchClass* c : active_) {
+                HIPCHK(hipStreamWaitEvent(c->ex->st, fork_, 0));
+                c->ex->issue();
+                HIPCHK(hipEventRecord(c->done, c->ex->st));
+                HIPCHK(hipStreamWaitEvent(bst_, c->done, 0));
+            }
+        } catch (...) {
+            (void)hipStreamEndCapture(bst_, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(bst_, &gr));
+        hipGraphExec_t ge = nullptr;
+        const hipError_t e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        HIPCHK(e);
+        seen_.erase(key);
+        whole_.emplace(std::move(key), ge);
+        ++captures_;
+        launch_whole(ge);
+    }
+    // bst_ waits for every class stream's work so far
+    void join_classes() {
+        for (auto& kv : classes_) {
+            HIPCHK(hipEventRecord(kv.second->done, kv.second->ex->st));
+            HIPCHK(hipStreamWaitEvent(bst_, kv.second->done, 0));
+        }
+    }
+    void launch_whole(hipGraphExec_t ge) {
+        join_classes();
+        HIPCHK(hipGraphLaunch(ge, bst_));
+        HIPCHK(hipEventRecord(after_, bst_));
+        for (auto& kv : classes_) HIPCHK(hipStreamWaitEvent(kv.second->ex->st, after_, 0));
+    }
+    void drop_whole() {
+        for (auto& kv : whole_) (void)hipGraphExecDestroy(kv.second);
+        whole_.clear();
+        seen_.clear();
+    }
+
+    static constexpr size_t kMaxWholeKeys = 256;
+    GraphMode mode_ = GRAPHS_WHOLE;
+    hipStream_t bst_ = nullptr;
+    hipEvent_t fork_ = nullptr, after_ = nullptr;
+    std::unordered_map<std::string, hipGraphExec_t> whole_;
+    std::unordered_set<std::string> seen_;
+    std::vector<BatchClass*> active_;
+    long long captures_ = 0, replays_ = 0, ticks_ = 0;
+    double host_us_ = 0.0;
     std::map<GridKey, std::unique_ptr<BatchClass>> classes_;
     std::map<int, std::unique_ptr<BatchMember>> members_;
-    std::deque<TickOps> ops_;
-    std::vector<TickOps*> ptrs_;
     int next_id_ = 0;
 };
 
@@ -219,7 +350,7 @@ inline double member_bytes(const cfg_grid& gr, const cfg_flags& f) {
 // Results come back through pinned host memory without stalling the others.
 class Server {
    public:
-    explicit Server(double budget) : budget_(budget) {}
+    explicit Server(double budget) : budget_(budget) { batch_.set_graphs(GRAPHS_PER_CLASS); }
     ~Server() {
         batch_.sync();
         for (auto& kv : done_)
@@ -482,7 +613,7 @@ int cfg_batch_cfl(cfg_batch* b, int member, double* cfl) {
 int cfg_batch_set_graphs(cfg_batch* b, int enable) {
     return guarded([&] {
         if (!b) throw Error(CFG_ERR_ARG, "null batch");
-        b->b.set_graphs(enable != 0);
+        b->b.set_graphs(enable);
     });
 }
 
@@ -490,6 +621,13 @@ int cfg_batch_graph_stats(const cfg_batch* b, long long* captures, long long* re
     return guarded([&] {
         if (!b) throw Error(CFG_ERR_ARG, "null batch");
         b->b.graph_stats(captures, replays);
+    });
+}
+
+int cfg_batch_tick_stats(const cfg_batch* b, long long* ticks, long long* graph_launches, double* host_us) {
+    return guarded([&] {
+        if (!b) throw Error(CFG_ERR_ARG, "null batch");
+        b->b.tick_stats(ticks, graph_launches, host_us);
     });
 }
 

@@ -1080,9 +1080,13 @@ __device__ void zc2r_pair(const ZCtx& c, const double2* __restrict__ xp, int fa,
     __syncthreads();
 }
 
-// r2c of phys[sa] (and phys[sb]) -> zo fields ga, gb (kz <= kzd kept)
+// r2c of phys[sa] (and phys[sb]) -> zo fields ga, gb (kz <= kzd kept).
+// fold != 0 (with sb >= 0): field ga receives A + fold i kz B instead of A,
+// kz = 2 pi k / Lz, A and B the spectra of phys[sa] and phys[sb]; gb < 0 then
+// drops B.
 template <int NZ>
-__device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int ga, int sb, int gb) {
+__device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int ga, int sb, int gb,
+                          double fold = 0.0) {
     const Geom& g = c.g;
     const int Nz = g.Nz, nkz = g.nkz;
     const double* pa = physf(c, sa);
@@ -1100,8 +1104,15 @@ __device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int g
         const double2 Zk = r[l * P + pidx(k)];
         const double2 Zn = r[l * P + pidx((Nz - k) % Nz)];
         const long base = ((long)c.y * g.Nx + c.x0 + l) * nkz + k;
-        zo[(long)ga * fsz + base] = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
-        if (gb >= 0) zo[(long)gb * fsz + base] = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
+        const double2 A = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
+        const double2 B = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
+        if (fold != 0.0 && pb) {
+            const double fk = fold * ((2 * PI / g.Lz) * k);
+            zo[(long)ga * fsz + base] = make_double2(A.x - fk * B.y, A.y + fk * B.x);
+        } else {
+            zo[(long)ga * fsz + base] = A;
+        }
+        if (gb >= 0) zo[(long)gb * fsz + base] = B;
     }
     __syncthreads();
 }
@@ -1109,7 +1120,8 @@ __device__ void zr2c_pair(const ZCtx& c, double2* __restrict__ zo, int sa, int g
 // physical-space products. xp fields: ROT u v w wx wy wz; CONV/SKEW u v w
 // du/dy dv/dy dw/dy du/dx dv/dx dw/dx; DIV u v w. zo outputs: g0..g2 (the
 // u.grad u / omega x u part and the rotation term) then uu00 uu01 uu02 uu11
-// uu12 uu22 for DIV/SKEW.
+// uu12 uu22 for DIV; for SKEW g_i + i kz u_i u_2 / 2 then uu00 uu01 uu02 uu11
+// uu12 (8 fields).
 template <int METHOD, int NZ>
 __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, const NLMember* __restrict__ tab,
                                                int want_g, const double2* __restrict__ xp0, double2* __restrict__ zo0,
@@ -1176,10 +1188,11 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, const
             fa[w] += su(u1, w) * pa[w];
             fa[w] += su(u2, w) * pb[w];
             if (rotation != 0.0) fa[w] -= rotation * u1[w];  // nse.cpp:61-80
-            if (SK) pb[w] = u0[w] * u0[w];
+            if (SK) pb[w] = u0[w] * u2[w];
         }
         __syncthreads();
-        zr2c_pair<NZ>(c, zo, 5, 0, SK ? 4 : -1, SK ? 3 : -1);  // f0 (and uu00)
+        // skew-symmetric: f0 + i kz u0u2 / 2 and u0u2 (the layout of k_zwave)
+        zr2c_pair<NZ>(c, zo, 5, 0, SK ? 4 : -1, SK ? 5 : -1, SK ? 0.5 : 0.0);
         zc2r_pair<NZ>(c, xp, 7, false, 3, 4, false, 4);  // dv/dx, dv/dy
         for (int w = threadIdx.x; w < npts; w += blockDim.x) {
             fa[w] = 0.0 + su(u0, w) * pa[w];
@@ -1197,23 +1210,27 @@ __global__ void __launch_bounds__(256) k_zpass(Geom g, FFTPlan pz, int XB, const
             fb[w] += su(u2, w) * pb[w];
         }
         __syncthreads();
-        zr2c_pair<NZ>(c, zo, 5, 1, 6, 2);  // f1, f2
-        if (SK) {  // remaining entries of outer(u, u) (diffops.cpp:2336-2388)
+        if (SK) {
+            // f1 + i kz u1u2 / 2 and u1u2, f2 + i kz u2u2 / 2, then the rest of
+            // outer(u, u) (diffops.cpp:2336-2388): u0u0, u0u1, u1u1
             for (int w = threadIdx.x; w < npts; w += blockDim.x) {
-                pa[w] = u0[w] * u1[w];
-                pb[w] = u0[w] * u2[w];
+                pa[w] = u1[w] * u2[w];
+                pb[w] = u2[w] * u2[w];
             }
             __syncthreads();
-            zr2c_pair<NZ>(c, zo, 3, 4, 4, 5);
+            zr2c_pair<NZ>(c, zo, 5, 1, 3, 7, 0.5);
+            zr2c_pair<NZ>(c, zo, 6, 2, 4, -1, 0.5);
             for (int w = threadIdx.x; w < npts; w += blockDim.x) {
-                pa[w] = u1[w] * u1[w];
-                pb[w] = u1[w] * u2[w];
+                pa[w] = u0[w] * u0[w];
+                pb[w] = u0[w] * u1[w];
             }
             __syncthreads();
-            zr2c_pair<NZ>(c, zo, 3, 6, 4, 7);
-            for (int w = threadIdx.x; w < npts; w += blockDim.x) pa[w] = u2[w] * u2[w];
+            zr2c_pair<NZ>(c, zo, 3, 3, 4, 4);
+            for (int w = threadIdx.x; w < npts; w += blockDim.x) pa[w] = u1[w] * u1[w];
             __syncthreads();
-            zr2c_pair<NZ>(c, zo, 3, 8, -1, -1);
+            zr2c_pair<NZ>(c, zo, 3, 6, -1, -1);
+        } else {
+            zr2c_pair<NZ>(c, zo, 5, 1, 6, 2);  // f1, f2
         }
     } else {
         zc2r_pair<NZ>(c, xp, 0, false, 0, 1, false, 1);
@@ -1498,10 +1515,12 @@ __device__ __forceinline__ void zw_c2r(const ZWCtx& c, const double2* __restrict
     }
 }
 
-// r2c of pa (and pb when TWO) -> zo fields ga (and gb), bins kz < nkz
+// r2c of pa (and pb when TWO) -> zo fields ga (and gb), bins kz < nkz.
+// fold != 0 (with TWO): field ga receives A + fold i kz B instead of A, with
+// kz = 2 pi k / Lz, A and B the spectra of pa and pb; gb < 0 then drops B.
 template <int NZ, bool TWO>
 __device__ __forceinline__ void zw_r2c(const ZWCtx& c, double2* __restrict__ zo, const double (&pa)[ZW<NZ>::N1],
-                                       const double (&pb)[ZW<NZ>::N1], int ga, int gb) {
+                                       const double (&pb)[ZW<NZ>::N1], int ga, int gb, double fold = 0.0) {
     constexpr int N1 = ZW<NZ>::N1, N2 = ZW<NZ>::N2, N = ZW<NZ>::N;
     const int j = c.gi;
     double2 v[N1];
@@ -1522,8 +1541,15 @@ __device__ __forceinline__ void zw_r2c(const ZWCtx& c, double2* __restrict__ zo,
             if (k < c.nkz) {
                 const double2 Zk = w[p];
                 const double2 Zn = c.T[(N - k) % N];
-                zo[(long)ga * c.fsz + c.base + k] = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
-                if (TWO) zo[(long)gb * c.fsz + c.base + k] = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
+                const double2 A = make_double2(0.5 * (Zk.x + Zn.x), 0.5 * (Zk.y - Zn.y));
+                const double2 B = make_double2(0.5 * (Zk.y + Zn.y), -0.5 * (Zk.x - Zn.x));
+                if (TWO && fold != 0.0) {
+                    const double fk = fold * (c.dk * k);
+                    zo[(long)ga * c.fsz + c.base + k] = make_double2(A.x - fk * B.y, A.y + fk * B.x);
+                } else {
+                    zo[(long)ga * c.fsz + c.base + k] = A;
+                }
+                if (TWO && gb >= 0) zo[(long)gb * c.fsz + c.base + k] = B;
             }
         }
     }
@@ -1583,6 +1609,13 @@ __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict
         // (diffops.cpp:3192-3199, 3626-3633). The skew-symmetric form keeps
         // h = u/2 only: halving is exact, so s u_j is h_j, u_j is 2 h_j and
         // u_i u_j is 4 (h_i h_j), all with the bits of the unscaled products.
+        //
+        // Skew-symmetric outputs: the z-derivative part of div(u u)/2
+        // (diffops.cpp:3254-3279) is added here, in z-spectral space:
+        // field i = f_i + 1/2 i kz (u_i u_z) (i = 0, 1, 2), then u_0u_0, u_0u_1,
+        // u_0u_2, u_1u_1, u_1u_2 in fields 3-7; u_2u_2 is needed only for that
+        // z-derivative and never leaves. The forward transforms add the x and y
+        // parts of the divergence.
         constexpr bool SK = (METHOD == NL_SKEW);
         constexpr double s = SK ? 0.5 : 1.0, us = SK ? 2.0 : 1.0;
         double h0[N1], h1[N1], h2[N1], pa[N1], pb[N1], fa[N1], fb[N1];
@@ -1604,9 +1637,10 @@ __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict
             fa[m] += h1[m] * pa[m];
             fa[m] += h2[m] * pb[m];
             if (rotation != 0.0) fa[m] -= rotation * (us * h1[m]);  // nse.cpp:61-80
-            if (SK) pb[m] = 4.0 * (h0[m] * h0[m]);
+            if (SK) pb[m] = 4.0 * (h0[m] * h2[m]);
         }
-        zw_r2c<NZ, SK>(c, zo, fa, pb, 0, 3);              // f0 (and uu00)
+        if constexpr (SK) zw_r2c<NZ, true>(c, zo, fa, pb, 0, 5, 0.5);  // f0 + i kz u0u2 / 2, and u0u2
+        else zw_r2c<NZ, false>(c, zo, fa, pb, 0, -1);                  // f0
         zw_c2r<NZ>(c, xp, 7, false, 4, false, pa, pb);   // dv/dx, dv/dy
 #pragma unroll
         for (int m = 0; m < N1; ++m) {
@@ -1626,23 +1660,25 @@ __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict
             fb[m] += h1[m] * pa[m];
             fb[m] += h2[m] * pb[m];
         }
-        zw_r2c<NZ, true>(c, zo, fa, fb, 1, 2);            // f1, f2
-        if constexpr (SK) {  // remaining entries of outer(u, u) (diffops.cpp:2336-2388)
+        if constexpr (SK) {
 #pragma unroll
-            for (int m = 0; m < N1; ++m) {
-                pa[m] = 4.0 * (h0[m] * h1[m]);
-                pb[m] = 4.0 * (h0[m] * h2[m]);
-            }
-            zw_r2c<NZ, true>(c, zo, pa, pb, 4, 5);
-#pragma unroll
-            for (int m = 0; m < N1; ++m) {
-                pa[m] = 4.0 * (h1[m] * h1[m]);
-                pb[m] = 4.0 * (h1[m] * h2[m]);
-            }
-            zw_r2c<NZ, true>(c, zo, pa, pb, 6, 7);
+            for (int m = 0; m < N1; ++m) pa[m] = 4.0 * (h1[m] * h2[m]);
+            zw_r2c<NZ, true>(c, zo, fa, pa, 1, 7, 0.5);   // f1 + i kz u1u2 / 2, and u1u2
 #pragma unroll
             for (int m = 0; m < N1; ++m) pa[m] = 4.0 * (h2[m] * h2[m]);
-            zw_r2c<NZ, false>(c, zo, pa, pa, 8, -1);
+            zw_r2c<NZ, true>(c, zo, fb, pa, 2, -1, 0.5);  // f2 + i kz u2u2 / 2
+            // remaining entries of outer(u, u) (diffops.cpp:2336-2388)
+#pragma unroll
+            for (int m = 0; m < N1; ++m) {
+                pa[m] = 4.0 * (h0[m] * h0[m]);
+                pb[m] = 4.0 * (h0[m] * h1[m]);
+            }
+            zw_r2c<NZ, true>(c, zo, pa, pb, 3, 4);
+#pragma unroll
+            for (int m = 0; m < N1; ++m) pa[m] = 4.0 * (h1[m] * h1[m]);
+            zw_r2c<NZ, false>(c, zo, pa, pa, 6, -1);
+        } else {
+            zw_r2c<NZ, true>(c, zo, fa, fb, 1, 2);            // f1, f2
         }
     } else {
         double u0[N1], u1[N1], u2[N1], pa[N1], pb[N1];
@@ -1916,7 +1952,8 @@ __global__ void __launch_bounds__(256) k_yfwd_t(Geom g, YMats ym, int have_g, co
 #pragma unroll
             for (int j = 0; j < 2; ++j) Acc[c][i][j] = make_double2(0.0, 0.0);
     for (int y0 = 0; y0 < Ny; y0 += YB_K) {
-        // G_i = g_i + beta (ikx uu_i0 + ikz uu_i2), H_i = beta uu_i1 (uu index -> fo field: 00:3 01:4 02:5 11:6 12:7 22:8)
+        // G_i = g_i + beta (ikx uu_i0 + ikz uu_i2), H_i = beta uu_i1 (uu index -> fo field: 00:3 01:4 02:5 11:6
+        // 12:7 22:8). The skew-symmetric z pass has already added beta ikz uu_i2 to g_i (and keeps no uu22).
         for (int i = threadIdx.x; i < 3 * YB_K * YB_M; i += 256) {
             const int mm = i % YB_M, kk = (i / YB_M) % YB_K, c = i / (YB_M * YB_K);
             const int y = y0 + kk, m = m0 + mm;
@@ -1929,8 +1966,8 @@ __global__ void __launch_bounds__(256) k_yfwd_t(Geom g, YMats ym, int have_g, co
                     const int uuf[3][3] = {{3, 4, 5}, {4, 6, 7}, {5, 7, 8}};
                     const double2 x0 = fo[((long)uuf[c][0] * Ny + y) * M + m];
                     const double2 x1 = fo[((long)uuf[c][1] * Ny + y) * M + m];
-                    const double2 x2 = fo[((long)uuf[c][2] * Ny + y) * M + m];
-                    const double2 d = cadd(imul(dx, x0), imul(dz, x2));
+                    double2 d = imul(dx, x0);
+                    if (METHOD == NL_DIV) d = cadd(d, imul(dz, fo[((long)uuf[c][2] * Ny + y) * M + m]));
                     G = make_double2(G.x + beta * d.x, G.y + beta * d.y);
                     H = make_double2(beta * x1.x, beta * x1.y);
                 }
@@ -2278,7 +2315,9 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
         sdx = sg * 2 * PI * kx_of(g, ikx) / g.Lx;
         sdz = sg * 2 * PI * ikz / g.Lz;
     }
-    // field of uu_cj in fo: 00:3 01:4 02:5 11:6 12:7 22:8
+    // field of uu_cj in fo: 00:3 01:4 02:5 11:6 12:7 22:8. The skew-symmetric
+    // z pass has already added beta ikz uu_c2 to g_c (and keeps no uu22).
+    constexpr bool ZTERM = (METHOD == NL_DIV);
     const int f0 = c == 0 ? 3 : (c == 1 ? 4 : 5), f1 = c == 0 ? 4 : (c == 1 ? 6 : 7), f2 = c == 0 ? 5 : (c == 1 ? 7 : 8);
     const long rowst = 2 * M;
     const double* __restrict__ pg = fo + 2 * ((long)c * Ny * M + mcol) + part;
@@ -2291,7 +2330,7 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
     double dlast = 0.0;
     auto gval = [&](int y) {
         double G = have_g ? pg[y * rowst] : 0.0;
-        if (DIVF) G += sdx * p0[y * rowst] + sdz * p2[y * rowst];
+        if (DIVF) G += ZTERM ? sdx * p0[y * rowst] + sdz * p2[y * rowst] : sdx * p0[y * rowst];
         return G;
     };
     double sgn[NS], dgn[NS], shn[NS], dhn[NS];
@@ -2631,9 +2670,13 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     launch_z_any(method, g, pz, n, tab, want_g, xp, zo, sx, tn, st);
     T.end(K_ZPASS, ev);
     ev = T.begin();
-    // forward x-FFT of the z-pass outputs that exist
-    if (want_g) launch_xfft(false, g, px, 0, 3, n, zo, sx, fo, sy, tn, st);
-    if (method == NL_DIV || method == NL_SKEW) {
+    // forward x-FFT of the z-pass outputs that exist: fields 0-2 (g) when
+    // want_g, fields 3-8 (u_i u_j) in the divergence form, 3-7 in the
+    // skew-symmetric one; one launch when both exist (the fields are
+    // contiguous)
+    const bool uu = method == NL_DIV || method == NL_SKEW;
+    if (want_g) launch_xfft(false, g, px, 0, method == NL_SKEW ? 8 : (uu ? 9 : 3), n, zo, sx, fo, sy, tn, st);
+    else if (uu) {
         const long off = 3L * g.Ny * g.Nx * g.nkz;
         const long offo = 3L * g.Ny * g.M;
         launch_xfft(false, g, px, 0, 6, n, zo + off, sx, fo + offo, sy, tn, st);

@@ -612,15 +612,23 @@ void Exec::issue() {
     HIPCHK(hipGetLastError());
 }
 
-void Exec::run(const std::vector<TickOps*>& ops) {
-    if (ops.empty()) return;
+void Exec::prepare(const std::vector<TickOps*>& ops) {
     reserve((int)ops.size());
     plan(ops);
+}
+
+void Exec::append_key(std::string& key) const {
+    key.append(reinterpret_cast<const char*>(recs_.data()), recs_.size() * sizeof(LaunchRec));
+}
+
+void Exec::run(const std::vector<TickOps*>& ops) {
+    if (ops.empty()) return;
+    prepare(ops);
     if (!graphs_on_ || kt.on) {  // timing needs the events between launches
         issue();
         return;
     }
-    const unsigned long gen = buf_gen_ * 1000003UL + cache.generation();
+    const unsigned long gen = generation();
     if (gen != graph_gen_) {
         HIPCHK(hipStreamSynchronize(st));  // replays in flight may hold the old graphs
         drop_graphs();

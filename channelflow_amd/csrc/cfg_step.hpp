@@ -186,6 +186,16 @@ class Exec {
     long graph_replays() const { return replays_; }
     long graph_captures() const { return captures_; }
 
+    // A tick planned without being launched, for graphs that span several
+    // Execs (Batch): prepare() makes the tables resident and fills the launch
+    // records, append_key() adds them to a graph key, issue() launches them on
+    // st (also inside a capture that st has joined). generation() changes when
+    // an address a recorded launch holds may have moved.
+    void prepare(const std::vector<TickOps*>& ops);
+    void append_key(std::string& key) const;
+    unsigned long generation() const { return buf_gen_ * 1000003UL + cache.generation(); }
+    void issue();  // launches recs_ on st
+
     std::shared_ptr<Geometry> geo;
     hipStream_t st = nullptr;
     DevPool pool;  // before everything that returns memory to it
@@ -198,7 +208,6 @@ class Exec {
     void reserve(int n);
     void plan(const std::vector<TickOps*>& ops);  // fills recs_, every table of one arena generation
     void plan_once(const std::vector<TickOps*>& ops);
-    void issue();                                  // launches recs_ on st
     void drop_graphs();
     int cap_ = 0;
     DBuf yp_, xp_, zo_, fo_, cflout_, nfout_;

@@ -83,3 +83,39 @@ def test_traffic_from_counters(tmp_path, fetch_ratio):
     assert d["_scale"]["write"] == pytest.approx(1.0)
     assert d["k_tau_solve"] == pytest.approx(1024 * (sf * 200.0 + 20.0))
     assert d["k_zpass"] == pytest.approx(1024 * (sf * 50.0 + 20.0))
+
+
+class _FakeBatch:
+    """Graph and tick counters of a batch whose first `eager` ticks run eagerly,
+    the next `captures` ticks capture a whole-batch graph, and every later tick
+    replays one."""
+
+    def __init__(self, eager, captures):
+        self.eager, self.cap, self.ticks, self.c, self.r = eager, captures, 0, 0, 0
+
+    def advance(self, n):
+        for _ in range(n):
+            self.ticks += 1
+            if self.ticks > self.eager + self.cap:
+                self.r += 1
+            elif self.ticks > self.eager:
+                self.c += 1
+
+    def graph_stats(self):
+        return self.c, self.r
+
+    def tick_stats(self):
+        return self.ticks, self.c + self.r, 0.0
+
+
+@pytest.mark.parametrize("eager,captures,expect", [(0, 0, 6), (4, 3, 18), (10, 9, 30)])
+def test_steady_state_ends_on_a_replay_only_block(eager, captures, expect):
+    """The bench's warm-up extension stops after the first block of steps that
+    captured nothing and replayed one graph per tick."""
+    b = _FakeBatch(eager, captures)
+    assert bench.steady_state(b, 1) == expect
+
+
+def test_steady_state_gives_up():
+    with pytest.raises(SystemExit):
+        bench.steady_state(_FakeBatch(1000, 0), 1)

@@ -53,7 +53,9 @@ def single(grid, fl, u0, n):
 
 def test_graph_replay_matches_eager(gpu):
     """Replayed hipGraphs give bit-identical states to eager launches, also
-    across a join and a removal mid-run (new launch sequences, new captures)."""
+    across a join and a removal mid-run (new launch sequences, new captures),
+    with one graph per grid class and with one graph per tick for the whole
+    batch; the whole-batch mode launches at most one graph per tick."""
     def run(graphs):
         b = cfa.Batch()
         b.set_graphs(graphs)
@@ -66,19 +68,24 @@ def test_graph_replay_matches_eager(gpu):
         late = b.add(fields(grid, u0), cfa.DNSFlags(**fl))
         b.advance(7)
         b.remove(ids[1])
-        b.advance(6)
+        b.advance(20)  # long enough for the whole batch's launch sequences to repeat (period 6)
         keep = [ids[0], ids[2], ids[3], late]
-        return {k: b.state(k) for k in keep}, b.graph_stats()
+        return {k: b.state(k) for k in keep}, b.graph_stats(), b.tick_stats(), b.size()[1]
 
-    eager, st_e = run(False)
-    graphed, st_g = run(True)
-    print(f"graph captures/replays: eager {st_e}, graphed {st_g}", flush=True)
-    for k in eager:
-        assert np.array_equal(eager[k][0], graphed[k][0]) and np.array_equal(eager[k][1], graphed[k][1])
-    # start-up substeps and every change of membership are new launch
-    # sequences; each repeating one is captured once and then replayed
-    assert st_e == (0, 0)
-    assert st_g[0] > 0 and st_g[1] > 0
+    eager, st_e, tk_e, _ = run(False)
+    for mode in (cfa.Batch.GRAPHS_PER_CLASS, cfa.Batch.GRAPHS_WHOLE):
+        graphed, st_g, tk_g, ncls = run(mode)
+        print(f"graph mode {mode}: captures/replays {st_g}, ticks/launches/host us {tk_g} ({ncls} classes); "
+              f"eager {st_e}", flush=True)
+        for k in eager:
+            assert np.array_equal(eager[k][0], graphed[k][0]) and np.array_equal(eager[k][1], graphed[k][1])
+        # start-up substeps and every change of membership are new launch
+        # sequences; each repeating one is captured once and then replayed
+        assert st_g[0] > 0 and st_g[1] > 0
+        assert tk_g[0] == tk_e[0]
+        if mode == cfa.Batch.GRAPHS_WHOLE:
+            assert ncls > 1 and tk_g[1] <= tk_g[0]
+    assert st_e == (0, 0) and tk_e[1] == 0
 
 
 def test_batch_cfl_matches_single(gpu):
