@@ -18,20 +18,27 @@ The same JSON line also carries
               horizons 100-2000 steps, admitted by HBM budget, with p50/p99
               completion latency and aggregate grid-point*steps/s, beside the
               same requests run one at a time on the plain path; `saturated`
-              repeats it with every request queued at t = 0 (the capacity);
+              repeats it with every request queued at t = 0 (the capacity),
+              `budget_4GB` the same under a 4 GB budget that binds
+              (admission with backfilling decides what runs at once);
   multishooting one call of the multiple-shooting batch client: the residual
               of 16 shots and 8 Jacobian products, 144 integrations of
               32x49x32 over T = 4 in one Batch, timed whole, beside the same
               integrations on the plain path;
   single_c2   the plain 192x129x192 Poiseuille channel (rotational, SBDF3,
               dt = 0.005) and its per-kernel times;
-  graphs      hipGraph captures and replays inside the timed region;
+  graphs      hipGraph captures and replays inside the timed region (one
+              graph per tick for the whole batch), graph launches and host
+              microseconds per tick, and the same steps with one graph per
+              grid class (`per_class`);
   roofline    the dominant kernel of the headline workload (HIP events over
               a profiled pass of the same members right after the timed
               region, one grid class at a time so that a launch's events
               time that kernel alone; replay is off while events are
               recorded);
-  cpu_baseline the oracle (oracle/cpu_ref.cpp, one core) on a bounded sample.
+  cpu_baseline the oracle (oracle/cpu_ref.cpp) on a bounded sample: one
+              member per grid class on one core, and one member per core on
+              all host cores, with the CPU model.
 """
 import argparse
 import json
@@ -321,6 +328,7 @@ def run_served(cfa, n, rate, seed, budget):
     lat = {}
     t0 = time.perf_counter()
     max_running = 0
+    max_used = 0.0
     next_report = 5.0
     while len(lat) < n:
         now = time.perf_counter() - t0
@@ -330,6 +338,7 @@ def run_served(cfa, n, rate, seed, budget):
             submitted += 1
         done, running, queued = srv.step()
         max_running = max(max_running, running)
+        max_used = max(max_used, srv.memory())
         now = time.perf_counter() - t0
         for rid in done:
             i = rid_of[rid]
@@ -348,7 +357,8 @@ def run_served(cfa, n, rate, seed, budget):
     lats = np.array([lat[i] for i in range(n)])
     return dict(value=work / wall, unit="grid-pt*steps/s", requests=n, arrival_rate_per_s=rate,
                 p50_latency_s=float(np.percentile(lats, 50)), p99_latency_s=float(np.percentile(lats, 99)),
-                wall_s=wall, max_concurrent=max_running, hbm_budget_bytes=budget, work_grid_pt_steps=float(work))
+                wall_s=wall, max_concurrent=max_running, hbm_budget_bytes=budget, max_admitted_bytes=max_used,
+                work_grid_pt_steps=float(work))
 
 
 def plain_for_requests(cfa, n, seed, steps, warmup):
@@ -616,6 +626,13 @@ def main():
         sat["arrival_rate_per_s"] = "all at t=0"
         out["served"]["saturated"] = sat
         print(f"bench: served saturated {json.dumps(sat)}", file=sys.stderr, flush=True)
+        # the same requests under a budget that binds: admission (with
+        # backfilling) decides how many run at once
+        print("bench: serving, every request queued at t=0, 4 GB budget", file=sys.stderr, flush=True)
+        tight = run_served(cfa, args.serve_requests, None, 4242, 4e9)
+        tight["arrival_rate_per_s"] = "all at t=0"
+        out["served"]["budget_4GB"] = tight
+        print(f"bench: served with a 4 GB budget {json.dumps(tight)}", file=sys.stderr, flush=True)
         if not args.no_plain:
             print("bench: served request mix on the plain path", file=sys.stderr, flush=True)
             sp = plain_for_requests(cfa, args.serve_requests, 4242, 20, 5)

@@ -210,6 +210,27 @@ class Batch {
     }
     int nclasses() const { return (int)classes_.size(); }
 
+    BatchClass* find_class(const cfg_grid& grid, bool dealias) {
+        auto it = classes_.find(grid_key(grid, dealias));
+        return it == classes_.end() ? nullptr : it->second.get();
+    }
+    // Device memory held for the integrations: every class's pool blocks (in
+    // use or free) and work buffers, and the tau sets of the members (each
+    // once). The classes' fixed tables are not counted.
+    size_t held_bytes() const {
+        size_t b = 0;
+        for (auto& kv : classes_) b += kv.second->ex->held_bytes();
+        std::unordered_set<const TauStore*> tau;
+        for (auto& kv : members_) kv.second->it->tau_sets(tau);
+        for (const TauStore* t : tau) b += t->bytes;
+        return b;
+    }
+    // a class left without members hands its free blocks and work buffers
+    // back to HIP
+    void trim_if_empty(BatchClass* c) {
+        if (c && c->members.empty()) c->ex->trim();
+    }
+
     // hipGraph capture and replay of repeating ticks (GraphMode; whole-batch
     // graphs by default)
     void set_graphs(int mode) {
@@ -329,19 +350,107 @@ class Batch {
     int next_id_ = 0;
 };
 
-// Device bytes one integration holds while admitted: its fields and
-// history, its share of the class work buffers, and its tau data (shared
-// when another member has the same grid, nu and dt; counted in full).
-inline double member_bytes(const cfg_grid& gr, const cfg_flags& f) {
-    const bool de = f.dealiasing == CFG_DealiasXZ;
-    const long kxd = de ? gr.Nx / 3 - 1 : gr.Nx / 2 - 1, kzd = de ? gr.Nz / 3 - 1 : gr.Nz / 2 - 1;
-    const double M = (double)(2 * kxd + 1) * (kzd + 1), L = gr.Ny * M;  // complex values per component
-    const double X = (double)gr.Ny * gr.Nx * (kzd + 1);
-    const double state = 16.0 * L * (4 + 4 * 4 + 3 * 4 + 3 * 3 + 3);  // cur, history, NL history, init buffers
-    const double work = 16.0 * (18 * L + 18 * X) + 8.0 * 3 * L;
-    const double tau = 8.0 * (TAU_NA * L + 6 * M) * 4;  // main + 3 init substeps
-    return state + work + tau;
+// The device memory of an integration: the pool blocks (doubles each) its
+// Integrator and steppers make (cfg_step.cpp: its state u and q; a multistep
+// scheme's history of u and q and of f; the three field buffers of a
+// CNAB-style or Runge-Kutta scheme, as the main one or to start a multistep
+// one; the mean pressure gradients of the main and start-up NSE) and its tau
+// sets (one lambda each: 1 for a multistep or CNAB2 scheme, 3 for SMRK2 and
+// CNRK2, plus those of the start-up scheme).
+struct MemberPlan {
+    std::vector<size_t> blocks;
+    int tau_sets = 0;
+};
+inline MemberPlan member_plan(long Ny, long M, const cfg_flags& f) {
+    const size_t nf = 2 * 3 * (size_t)Ny * M, ns = 2 * (size_t)Ny * M;
+    MemberPlan p;
+    p.blocks = {nf, ns, 2, 2};
+    auto three = [&] {
+        for (int i = 0; i < 3; ++i) p.blocks.push_back(nf);
+    };
+    int order = 0;
+    switch (f.timestepping) {
+        case CFG_SBDF1: case CFG_CNFE1: order = 1; break;
+        case CFG_SBDF2: order = 2; break;
+        case CFG_SBDF3: order = 3; break;
+        case CFG_SBDF4: order = 4; break;
+        case CFG_CNAB2: three(); p.tau_sets = 1; break;
+        default: three(); p.tau_sets = 3; break;  // SMRK2, CNRK2
+    }
+    if (order > 0) {
+        p.tau_sets = 1;
+        for (int j = 0; j < order; ++j) {
+            if (order > 1) {
+                p.blocks.push_back(nf);
+                p.blocks.push_back(ns);
+            }
+            p.blocks.push_back(nf);
+        }
+        if (order > 1) {  // the start-up scheme
+            three();
+            p.tau_sets += f.initstepping == CFG_CNAB2 ? 1 : 3;
+        }
+    }
+    return p;
 }
+
+// Plans admissions against the device memory a batch holds for its
+// integrations: a request needs the pool blocks its class's free lists cannot
+// supply, its tau sets (counted in full, though members with the same grid,
+// nu and dt share them) and the growth of its class's work buffers
+// (Exec::grown_cap). A class's fixed tables (launch-table arena, geometry)
+// are outside the budget.
+class AdmissionPlan {
+   public:
+    // b null: plan against an empty batch (the request alone)
+    explicit AdmissionPlan(Batch* b) : batch_(b), held_(b ? b->held_bytes() : 0) {}
+    double held() const { return (double)held_; }
+    double committed() const { return (double)(held_ + pending_); }
+    size_t need(const cfg_grid& gr, const cfg_flags& f, bool commit) {
+        const bool de = f.dealiasing == CFG_DealiasXZ;
+        const long kxd = de ? gr.Nx / 3 - 1 : gr.Nx / 2 - 1, kzd = de ? gr.Nz / 3 - 1 : gr.Nz / 2 - 1;
+        Geom g{};
+        g.Nx = gr.Nx;
+        g.Ny = gr.Ny;
+        g.Nz = gr.Nz;
+        g.nkx = (int)(2 * kxd + 1);
+        g.nkz = (int)(kzd + 1);
+        g.M = g.nkx * g.nkz;
+        const GridKey k = grid_key(gr, de);
+        BatchClass* cls = batch_ ? batch_->find_class(gr, de) : nullptr;
+        size_t bytes = 0;
+        const MemberPlan mp = member_plan(g.Ny, g.M, f);
+        std::map<size_t, int> use;
+        for (size_t n : mp.blocks) {
+            const int avail = cls ? (int)cls->ex->pool.free_count(n) - claimed_[{k, n}] - use[n] : 0;
+            if (avail > 0) ++use[n];
+            else bytes += device_bytes_taken(n * sizeof(double));
+        }
+        bytes += (size_t)mp.tau_sets * tau_set_bytes(g);
+        bytes += kMemberTables;
+        auto cit = cap_.find(k);
+        const int cap0 = cit != cap_.end() ? cit->second : (cls ? cls->ex->cap() : 0);
+        const int n0 = members_.count(k) ? members_[k] : (cls ? (int)cls->members.size() : 0);
+        const int cap1 = Exec::grown_cap(cap0, n0 + 1);
+        bytes += Exec::work_bytes(g, cap1) - Exec::work_bytes(g, cap0);
+        if (commit) {
+            for (auto& kv : use) claimed_[{k, kv.first}] += kv.second;
+            cap_[k] = cap1;
+            members_[k] = n0 + 1;
+            pending_ += bytes;
+        }
+        return bytes;
+    }
+
+   private:
+    // the small device tables of an integration's model (base-flow values and
+    // constants of its solves), each its own allocation
+    static constexpr size_t kMemberTables = size_t(64) << 10;
+    Batch* batch_;
+    size_t held_, pending_ = 0;
+    std::map<std::pair<GridKey, size_t>, int> claimed_;
+    std::map<GridKey, int> cap_, members_;
+};
 
 // Continuous batching of integration requests: queued requests are admitted
 // while the device-memory budget allows, every admitted request advances one
@@ -363,7 +472,7 @@ class Server {
         check_flags(f);
         check_grid(g);
         if (nsteps < 0) throw Error(CFG_ERR_ARG, "negative horizon");
-        const double bytes = member_bytes(g, f);
+        const double bytes = (double)AdmissionPlan(nullptr).need(g, f, false);
         if (bytes > budget_) throw Error(CFG_ERR_ARG, "request does not fit the device-memory budget");
         Req r;
         r.id = next_id_++;
@@ -427,7 +536,9 @@ class Server {
                 throw Error(CFG_ERR_NONFINITE, "request " + std::to_string(id) + ": velocity not finite at its horizon");
     }
 
-    double used() const { return used_; }
+    // device memory the server holds (its batch's pools, work buffers, table
+    // arenas and the tau data)
+    double used() const { return (double)batch_.held_bytes(); }
 
    private:
     struct Req {
@@ -451,16 +562,38 @@ class Server {
         bool ready = false, reported = false;
     };
 
+    // Admission in arrival order, with backfilling: when the request at the
+    // head of the queue does not fit the free budget, later requests that fit
+    // are admitted past it, until the head has waited kMaxHeadWait server
+    // steps; from then on nothing passes it, so running requests drain until
+    // it fits (a large request is delayed by small ones at most that long).
+    // What fits is planned against the memory the batch holds now plus what
+    // the requests admitted in this step will take (AdmissionPlan).
     void admit() {
-        while (!queue_.empty() && used_ + queue_.front().bytes <= budget_) {
-            Req r = std::move(queue_.front());
-            queue_.pop_front();
-            const int mid = batch_.add(r.grid, r.flags, r.u.data(), r.q.empty() ? nullptr : r.q.data());
-            batch_.get(mid).target = r.nsteps;
-            running_.emplace(r.id, Run{mid, r.bytes});
-            used_ += r.bytes;
+        if (queue_.empty()) return;
+        AdmissionPlan plan(&batch_);
+        auto fits = [&](const Req& r) {
+            return plan.committed() + (double)plan.need(r.grid, r.flags, false) <= budget_;
+        };
+        if (!fits(queue_.front())) ++head_wait_;
+        for (auto it = queue_.begin(); it != queue_.end();) {
+            const bool head = it == queue_.begin();
+            if (fits(*it)) {
+                Req r = std::move(*it);
+                it = queue_.erase(it);
+                plan.need(r.grid, r.flags, true);
+                const int mid = batch_.add(r.grid, r.flags, r.u.data(), r.q.empty() ? nullptr : r.q.data());
+                batch_.get(mid).target = r.nsteps;
+                running_.emplace(r.id, Run{mid, r.bytes});
+                if (head) head_wait_ = 0;
+                continue;
+            }
+            if (head_wait_ > kMaxHeadWait) break;  // the head has waited long enough: no more passing
+            ++it;
         }
     }
+    static constexpr long kMaxHeadWait = 256;
+    long head_wait_ = 0;  // server steps the current head of the queue has been blocked
 
     void retire(long long id, const Run& run) {
         BatchMember& m = batch_.get(run.member);
@@ -476,13 +609,17 @@ class Server {
         HIPCHK(hipMemcpyAsync(d.host + d.nu, s.q, d.nq * sizeof(double), hipMemcpyDeviceToHost, ex.st));
         HIPCHK(hipEventCreateWithFlags(&d.ev, hipEventDisableTiming));
         HIPCHK(hipEventRecord(d.ev, ex.st));
+        BatchClass* cls = m.cls;
         batch_.remove(run.member);  // buffers go back to the class pool, reused only by later work
-        used_ -= run.bytes;
+        batch_.trim_if_empty(cls);  // after the copy above (trim synchronizes the class stream)
+        // tau sets the request alone used go back to HIP now (hipFree waits for
+        // the kernels that used them), so the budget's count stays the device's
+        deferred_free(nullptr, 0, true);
         done_.emplace(id, d);
     }
 
     Batch batch_;
-    double budget_, used_ = 0;
+    double budget_;
     std::deque<Req> queue_;
     std::map<long long, Run> running_;
     std::map<long long, Done> done_;

@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include <string>
 #include <tuple>
+#include <unordered_set>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -440,11 +441,20 @@ inline void init_device_limits() {
     done = true;
 }
 
+// Device memory a hipMalloc of `bytes` takes (hipMemGetInfo deltas measured
+// on the MI355X): more than 1 MiB rounds up to whole 2 MiB; smaller blocks are
+// packed into 2 MiB chunks at up to 1.2x their size (1.25x counted here).
+inline size_t device_bytes_taken(size_t bytes) {
+    const size_t M1 = size_t(1) << 20, M2 = size_t(2) << 20;
+    if (bytes > M1) return (bytes + M2 - 1) / M2 * M2;
+    return (bytes + bytes / 4 + 4095) / 4096 * 4096;
+}
+
 // ---------------------------------------------------------------------------
 // Device memory recycled by exact size. One pool serves one stream: a block
 // released while kernels on that stream may still use it is only handed out
 // again for work queued later on the same stream, so stream order keeps it
-// safe. Blocks go back to HIP when the pool is destroyed.
+// safe. Blocks go back to HIP when the pool is destroyed or trimmed.
 // ---------------------------------------------------------------------------
 class DevPool {
    public:
@@ -463,7 +473,8 @@ class DevPool {
         if (fl.empty()) {
             double* p = nullptr;
             HIPCHK(hipMalloc(&p, n * sizeof(double)));
-            allocs_.push_back(p);
+            allocs_.insert(p);
+            held_ += device_bytes_taken(n * sizeof(double));
             return p;
         }
         double* p = fl.back();
@@ -473,10 +484,29 @@ class DevPool {
     void give(double* p, size_t n) {
         if (p) free_[n].push_back(p);
     }
+    // device bytes of every block the pool holds (in use or free)
+    size_t held() const { return held_; }
+    // free blocks of n doubles, for a caller that plans allocations
+    size_t free_count(size_t n) const {
+        auto it = free_.find(n);
+        return it == free_.end() ? 0 : it->second.size();
+    }
+    // returns the free blocks to HIP; the caller has synchronized the stream
+    void trim() {
+        for (auto& kv : free_) {
+            for (double* p : kv.second) {
+                (void)hipFree(p);
+                allocs_.erase(p);
+                held_ -= device_bytes_taken(kv.first * sizeof(double));
+            }
+        }
+        free_.clear();
+    }
 
    private:
     std::map<size_t, std::vector<double*>> free_;
-    std::vector<double*> allocs_;  // every allocation, freed with the pool
+    std::unordered_set<double*> allocs_;  // every allocation, freed with the pool
+    size_t held_ = 0;
 };
 
 // Device copy of a small constant table, shared by every user with the same
@@ -604,9 +634,13 @@ inline std::shared_ptr<Geometry> geometry_for(const cfg_grid& gr, bool dealias) 
 // Device memory whose release waits: hipFree synchronises the whole device, so
 // buffers that are dropped while other integrations run (tau data of a retired
 // request) are collected here and freed together once enough has piled up.
+inline size_t& deferred_pending_bytes() {
+    static size_t total = 0;
+    return total;
+}
 inline void deferred_free(double* p, size_t bytes, bool flush_all = false) {
     static std::vector<double*> pending;
-    static size_t total = 0;
+    size_t& total = deferred_pending_bytes();
     if (p) {
         pending.push_back(p);
         total += bytes;
@@ -617,6 +651,7 @@ inline void deferred_free(double* p, size_t bytes, bool flush_all = false) {
         total = 0;
     }
 }
+
 
 // Tau data of one (grid, nu, lambda_t, taucorrection), shared by every field
 // that uses it (NSE::reset_lambda, nse.cpp:673-705, builds one per field).
@@ -637,6 +672,13 @@ struct TauStore {
     }
 };
 
+// device bytes of one tau set (one lambda) of a grid
+inline size_t tau_set_bytes(const Geom& g) {
+    const size_t prof = (size_t)g.Ny * g.M;
+    return device_bytes_taken((size_t)TAU_NA * prof * sizeof(double)) +
+           device_bytes_taken(6 * (size_t)g.M * sizeof(double));
+}
+
 inline std::shared_ptr<TauStore> tau_for(const Geometry& geo, double nu, double lambda_t, bool taucorr,
                                          hipStream_t st, DevPool& pool, KTimer* kt) {
     using Key = std::tuple<GridKey, double, double, int>;
@@ -654,16 +696,15 @@ inline std::shared_ptr<TauStore> tau_for(const Geometry& geo, double nu, double 
     }
     auto ts = std::make_shared<TauStore>();
     const size_t prof = (size_t)g.Ny * g.M;
-    ts->bytes = ((size_t)TAU_NA * prof + 6 * (size_t)g.M) * sizeof(double);
     HIPCHK(hipMalloc(&ts->coef, (size_t)TAU_NA * prof * sizeof(double)));
     HIPCHK(hipMalloc(&ts->scal, 6 * (size_t)g.M * sizeof(double)));
+    ts->bytes = tau_set_bytes(g);
     ts->set.coef = ts->coef;
     ts->set.scal = ts->scal;
     ts->set.lambda_t = lambda_t;
     {
-        PBuf scratch(pool, 2 * prof);  // back to the pool: reused only by later work on st
         hipEvent_t ev = kt ? kt->begin() : nullptr;
-        launch_tau_setup(g, ts->set, nu, taucorr, scratch.p, st);
+        launch_tau_setup(g, ts->set, nu, taucorr, st);
         if (kt) kt->end(K_SETUP, ev);
         HIPCHK(hipGetLastError());
     }

@@ -303,18 +303,61 @@ __device__ TauView make_view(const Geom& g, const TauSet& s, int m, double nu, b
     return t;
 }
 
-__global__ void __launch_bounds__(64) k_tau_setup(Geom g, TauSet s, double nu, int taucorr, double* scratch) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= g.M) return;
-    const long S = g.M;
-    TauView t = make_view(g, s, m, nu, taucorr != 0);
-    SV s0{scratch + m, S}, s1{scratch + S * g.Ny + m, S};
-    tau_setup(t, s0, s1);
+// TauSolver constructor per mode (tausolver.cpp:81-176), T modes per block,
+// one thread each, with the mode's working set in LDS: its 12 tau profiles
+// (array a at element (a*Ny + n)*T + t), 6 scalars and 2 scratch profiles.
+// The arithmetic is tau_setup's, unchanged (the sequential set-up, bit for bit
+// the oracle's); only the memory it runs in moves from HBM to LDS: the
+// recurrences store and reload their profiles, which is an LDS round trip
+// here instead of an HBM one. The profiles then go to the mode-major tau set
+// with contiguous writes.
+__global__ void k_tau_setup(Geom g, TauSet s, double nu, int taucorr) {
+    extern __shared__ double sl[];
+    const int T = blockDim.x, t = threadIdx.x;
+    const int m0 = blockIdx.x * T, m = m0 + t;
+    const int Ny = g.Ny;
+    const long arr = (long)Ny * T;  // one profile of the block's modes
+    if (m < g.M) {
+        TauView v = make_view(g, s, m, nu, taucorr != 0);
+        auto A = [&](int a) { return SV{sl + a * arr + t, T}; };
+        v.ph.band = A(0);
+        v.ph.up = A(1);
+        v.ph.invd = A(2);
+        v.vh.band = A(3);
+        v.vh.up = A(4);
+        v.vh.invd = A(5);
+        v.Pp = A(6);
+        v.Pm = A(7);
+        v.vp = A(8);
+        v.vm = A(9);
+        v.P0 = A(10);
+        v.v0 = A(11);
+        v.scal = sl + 14 * arr + t;
+        v.sstride = T;
+        tau_setup(v, A(12), A(13));
+    }
+    __syncthreads();
+    const int nm = min(T, g.M - m0);
+    __attribute__((address_space(1))) double* coef = gmem_w(s.coef) + (long)m0 * TAU_NA * Ny;
+    for (int i = t; i < nm * TAU_NA * Ny; i += T) {
+        const int mm = i / (TAU_NA * Ny), r = i - mm * (TAU_NA * Ny), a = r / Ny, n = r - a * Ny;
+        coef[i] = sl[a * arr + (long)n * T + mm];
+    }
+    __attribute__((address_space(1))) double* scal = gmem_w(s.scal) + (long)m0 * 6;
+    for (int i = t; i < nm * 6; i += T) scal[i] = sl[14 * arr + (long)(i % 6) * T + i / 6];
 }
 
-void launch_tau_setup(const Geom& g, const TauSet& s, double nu, bool taucorr, double* scratch, hipStream_t st) {
-    const int blk = 64;
-    k_tau_setup<<<(g.M + blk - 1) / blk, blk, 0, st>>>(g, s, nu, taucorr ? 1 : 0, scratch);
+// modes per block of the set-up: as many as 64 KB of LDS hold (14 profiles
+// and 6 scalars each), at most 64
+static int tau_setup_modes(int Ny) {
+    const long per = (14L * Ny + 6) * (long)sizeof(double);
+    return (int)std::max(1L, std::min(64L, (64L * 1024) / per));
+}
+
+void launch_tau_setup(const Geom& g, const TauSet& s, double nu, bool taucorr, hipStream_t st) {
+    const int T = tau_setup_modes(g.Ny);
+    const size_t lds = (size_t)(14L * g.Ny + 6) * T * sizeof(double);
+    k_tau_setup<<<(g.M + T - 1) / T, T, lds, st>>>(g, s, nu, taucorr ? 1 : 0);
 }
 
 __device__ __forceinline__ CV cview(double* base, const Geom& g, int comp, int m) {
@@ -329,22 +372,32 @@ __device__ __forceinline__ CV cview(double* base, const Geom& g, int comp, int m
 // solve_P_and_v for the real and imaginary parts at once (NR = 2 right-hand
 // sides per scan); per part the same arithmetic as the one-RHS routines
 // (tausolver.cpp:193-251).
+// Ry is read again from its LDS row (re, im per coefficient) where it is
+// needed, instead of being held in registers through the pressure solve.
 template <int K, int W>
 __device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, gcdouble cm, gcdouble sc, double kappa2,
                                                   double lambda, double nu, bool taucorr, const WV<K> (&r)[2],
-                                                  const WV<K> (&Ry)[2], WV<K> (&P)[2], WV<K> (&v)[2]) {
+                                                  const double2* Ryrow, WV<K> (&P)[2], WV<K> (&v)[2]) {
     const long Ny = c.N;
     const double z2[2] = {0.0, 0.0};
     whelm_n<K, W, 2>(r, cm + 0 * Ny, cm + 1 * Ny, cm + 2 * Ny, kappa2, z2, z2, c, P);
     WV<K> tmp[2];
     wdiff_n<K, W, 2>(P, c, tmp);
+    double rtop[2], rtop1[2];  // Ry's top coefficients, for the tau correction
+    {
+        WV<K> Ry[2];
+        wload_c<K, W>(Ryrow, c, Ry[0], Ry[1]);
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+        for (int p = 0; p < 2; ++p) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            tmp[p].e[k] -= Ry[p].e[k];
-            tmp[p].o[k] -= Ry[p].o[k];
+            for (int k = 0; k < K; ++k) {
+                tmp[p].e[k] -= Ry[p].e[k];
+                tmp[p].o[k] -= Ry[p].o[k];
+            }
+            rtop[p] = wtop<K, W>(Ry[p], c);
+            rtop1[p] = wtop1<K, W>(Ry[p], c);
         }
+    }
     whelm_n<K, W, 2>(tmp, cm + 3 * Ny, cm + 4 * Ny, cm + 5 * Ny, lambda, z2, z2, c, v);
     // influence-matrix correction (tausolver.cpp:178-191)
     double dvb[2], dva[2];
@@ -371,8 +424,8 @@ __device__ __forceinline__ void tau_P_and_v_wave2(const WCtx& c, gcdouble cm, gc
     const WV<K> P0 = wload<K, W>(cm + 10 * Ny, c), v0 = wload<K, W>(cm + 11 * Ny, c);
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
-        double sigma1_Nb = lambda * wtop<K, W>(v[p], c) - nu * 0.0 - wtop<K, W>(Ry[p], c);
-        double sigma1_Nb1 = lambda * wtop1<K, W>(v[p], c) - nu * 0.0 - wtop1<K, W>(Ry[p], c);
+        double sigma1_Nb = lambda * wtop<K, W>(v[p], c) - nu * 0.0 - rtop[p];
+        double sigma1_Nb1 = lambda * wtop1<K, W>(v[p], c) - nu * 0.0 - rtop1[p];
         sigma1_Nb1 += (4.0 / c.L) * Nb * wtop<K, W>(P[p], c);
         const double sigmaNb = sigma1_Nb / (1.0 - sc[5]);
         const double sigmaNb1 = sigma1_Nb1 / (1.0 - sc[4]);
@@ -437,7 +490,7 @@ __device__ __forceinline__ void tau_stage_rhs(const TauMember& me, const Geom& g
 
 // waves per SIMD a tau-solve variant is compiled for: two 16-lane coefficient
 // blocks fit in the 168 registers of three waves
-constexpr int tau_waves(int K, int W) { return (K == 2 && W == 16) ? 3 : 1; }
+constexpr int tau_waves(int K, int W) { return (W == 16 && K == 2) ? 3 : 1; }
 constexpr int NXCD = 8;
 
 // Block order of a launch over (mode block, member) pairs: the dispatcher
@@ -508,76 +561,87 @@ k_tau_solve_wave(Geom g, const double* __restrict__ hb, int nmem, const TauMembe
     const double kx2pi = 2 * PI * kx / g.Lx, kz2pi = 2 * PI * kz / g.Lz;
     const gcdouble cm = gmem(s.coef) + (long)(valid ? m : 0) * TAU_NA * Ny;
     const gcdouble sc = gmem(s.scal) + (long)(valid ? m : 0) * 6;
-    WV<K> Rxr, Rxi, Ryr, Ryi, Rzr, Rzi;
-    if (valid) {
-        wload_c<K, W>(lds + (w * 3 + 0) * Ny, c, Rxr, Rxi);
-        wload_c<K, W>(lds + (w * 3 + 1) * Ny, c, Ryr, Ryi);
-        wload_c<K, W>(lds + (w * 3 + 2) * Ny, c, Rzr, Rzi);
-        if (zero) {  // kx = kz = 0 (nse.cpp:505-527): nu U'', nu W'' and the mean pressure gradient
-            const WV<K> Uyy = wload<K, W>(zc.nuUbaseyy, c), Wyy = wload<K, W>(zc.nuWbaseyy, c);
+    // LDS rows of this mode: Rx, Ry, Rz as staged. Each is read where it is
+    // needed rather than held in registers across the solves (the registers
+    // bound the waves per SIMD); the rows are then reused for the outputs.
+    double2* const Sx = lds + (w * 3 + 0) * Ny;
+    double2* const Sy = lds + (w * 3 + 1) * Ny;
+    double2* const Sz = lds + (w * 3 + 2) * Ny;
+    // Rx or Rz of this mode; kx = kz = 0 (nse.cpp:505-527) adds nu U'' (nu W'')
+    // and, with a fixed pressure gradient, the mean gradient
+    auto load_R = [&](const double2* S, const gcdouble base, double dPref, WV<K>& re, WV<K>& im) {
+        wload_c<K, W>(S, c, re, im);
+        if (zero) {
+            const WV<K> Byy = wload<K, W>(base, c);
             for (int k = 0; k < K; ++k) {
-                Rxr.e[k] += Uyy.e[k];
-                Rxr.o[k] += Uyy.o[k];
-                Rzr.e[k] += Wyy.e[k];
-                Rzr.o[k] += Wyy.o[k];
+                re.e[k] += Byy.e[k];
+                re.o[k] += Byy.o[k];
             }
-            if (!zc.bulk && c.lane == 0) {
-                Rxr.e[0] -= zc.dPdxRef;
-                Rzr.e[0] -= zc.dPdzRef;
-            }
+            if (!zc.bulk && c.lane == 0) re.e[0] -= dPref;
         }
-    }
-    __syncthreads();
-    WV<K> Pr, Pi;
+    };
+    const gcdouble nuUyy = gmem(zc.nuUbaseyy), nuWyy = gmem(zc.nuWbaseyy);
+    WV<K> Pr, Pi, vr, vi;
     if (valid) {
-        WV<K> vr, vi;
         if (zero) {  // solve_P_and_v for kx = kz = 0: P from the pressure Helmholtz, v = 0
+            WV<K> Ryr, Ryi;
+            wload_c<K, W>(Sy, c, Ryr, Ryi);
             Pr = whelm<K, W>(wdiff<K, W>(Ryr, c), cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
-            Pi = whelm<K, W>(wdiff<K, W>(Ryi, c), cm, cm + Ny, cm + 2L * Ny, kappa2, 0.0, 0.0, c);
             vr = wzero<K, W>();
             vi = wzero<K, W>();
+            Pi = wzero<K, W>();  // real parts only (nse.cpp:555-563)
         } else {  // real and imaginary parts together
-            const WV<K> Ry2[2] = {Ryr, Ryi};
             WV<K> rr[2], P2[2], v2[2];
-            wdiff_n<K, W, 2>(Ry2, c, rr);
-            for (int k = 0; k < K; ++k) {
-                rr[0].e[k] -= kx2pi * Rxi.e[k] + kz2pi * Rzi.e[k];
-                rr[0].o[k] -= kx2pi * Rxi.o[k] + kz2pi * Rzi.o[k];
-                rr[1].e[k] += kx2pi * Rxr.e[k] + kz2pi * Rzr.e[k];
-                rr[1].o[k] += kx2pi * Rxr.o[k] + kz2pi * Rzr.o[k];
+            {
+                WV<K> Ry2[2];
+                wload_c<K, W>(Sy, c, Ry2[0], Ry2[1]);
+                wdiff_n<K, W, 2>(Ry2, c, rr);
             }
-            tau_P_and_v_wave2<K, W>(c, cm, sc, kappa2, lambda, nu, taucorr != 0, rr, Ry2, P2, v2);
+            {
+                WV<K> Rxr, Rxi, Rzr, Rzi;
+                wload_c<K, W>(Sx, c, Rxr, Rxi);
+                wload_c<K, W>(Sz, c, Rzr, Rzi);
+                for (int k = 0; k < K; ++k) {
+                    rr[0].e[k] -= kx2pi * Rxi.e[k] + kz2pi * Rzi.e[k];
+                    rr[0].o[k] -= kx2pi * Rxi.o[k] + kz2pi * Rzi.o[k];
+                    rr[1].e[k] += kx2pi * Rxr.e[k] + kz2pi * Rzr.e[k];
+                    rr[1].o[k] += kx2pi * Rxr.o[k] + kz2pi * Rzr.o[k];
+                }
+            }
+            tau_P_and_v_wave2<K, W>(c, cm, sc, kappa2, lambda, nu, taucorr != 0, rr, Sy, P2, v2);
             Pr = P2[0];
             Pi = P2[1];
             vr = v2[0];
             vi = v2[1];
         }
-        if (zero) {  // real parts only (nse.cpp:555-563)
-            Pi = wzero<K, W>();
-            vi = wzero<K, W>();
-        }
-        wstore_c<K, W>(lds + (w * 3 + 0) * Ny, c, Pr, Pi);
-        wstore_c<K, W>(lds + (w * 3 + 1) * Ny, c, vr, vi);
+        wstore_c<K, W>(Sy, c, vr, vi);  // Ry's row is free: v goes out through it
     }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < 2 * Ny * MPB; idx += nthr) {
-        const int ww = idx % MPB, rest = idx / MPB, n = rest % Ny, slot = rest / Ny;
-        const int mm = m0 + ww;
-        if (mm < g.M) {
-            const double2 v = lds[(ww * 3 + slot) * Ny + n];
-            if (slot == 0) q[(long)n * g.M + mm] = dvec2{v.x, v.y};
-            else u[((long)1 * Ny + n) * g.M + mm] = dvec2{v.x, v.y};
+    // rows [slot][n] of the block's modes -> global component array dst (stride M)
+    auto drain = [&](int slot, __attribute__((address_space(1))) dvec2* dst) {
+        for (int idx = threadIdx.x; idx < Ny * MPB; idx += nthr) {
+            const int ww = idx % MPB, n = idx / MPB;
+            const int mm = m0 + ww;
+            if (mm < g.M) {
+                const double2 v = lds[(ww * 3 + slot) * Ny + n];
+                dst[(long)n * g.M + mm] = dvec2{v.x, v.y};
+            }
         }
-    }
+    };
     __syncthreads();
+    drain(1, u + (long)Ny * g.M);  // v
+    __syncthreads();
+    if (valid) wstore_c<K, W>(Sy, c, Pr, Pi);
+    __syncthreads();
+    drain(1, q);  // q = P
     if (valid) {
         const gcdouble vb = cm + 3L * Ny;
         const gcdouble vu = cm + 4L * Ny;
         const gcdouble vd = cm + 5L * Ny;
-        // u (slot 0) and w (slot 1) from the velocity Helmholtz operator. Called
+        // u (row Sx) and w (row Sz) from the velocity Helmholtz operator. Called
         // with explicit register arrays: selecting between them at run time
         // would put them in scratch memory.
-        auto solve_uw = [&](double k2, const WV<K>& Rr, const WV<K>& Ri, double umean, double* mu_out, int slot) {
+        auto solve_uw = [&](double k2, const WV<K>& Rr, const WV<K>& Ri, double umean, double* mu_out,
+                            double2* row) {
             WV<K> fr, fi;
             for (int k = 0; k < K; ++k) {
                 fr.e[k] = -(k2 * Pi.e[k]) - Rr.e[k];
@@ -586,45 +650,50 @@ k_tau_solve_wave(Geom g, const double* __restrict__ hb, int nmem, const TauMembe
                 fi.o[k] = k2 * Pr.o[k] - Ri.o[k];
             }
             WV<K> xr, xi;
-            if (zero && zc.bulk) {  // bulk-velocity constraint (tausolver.cpp:404-450)
+            if (zc.bulk) {  // bulk-velocity constraint (tausolver.cpp:404-450)
                 const double mu = whelm_mean<K, W>(fr, vb, vu, vd, lambda, nu, umean, c, xr);
                 if (c.lane == 0) *mu_out = mu;
             } else {
                 xr = whelm<K, W>(fr, vb, vu, vd, lambda, 0.0, 0.0, c);
             }
-            xi = zero ? wzero<K, W>() : whelm<K, W>(fi, vb, vu, vd, lambda, 0.0, 0.0, c);
-            wstore_c<K, W>(lds + (w * 3 + slot) * Ny, c, xr, xi);
+            xi = wzero<K, W>();
+            wstore_c<K, W>(row, c, xr, xi);
         };
-        if (zero) {
-            solve_uw(kx2pi, Rxr, Rxi, zc.umean, zc.dPdx_out, 0);
-            solve_uw(kz2pi, Rzr, Rzi, zc.wmean, zc.dPdz_out, 1);
+        if (zero) {  // kx = kz = 0: real parts only
+            {
+                WV<K> Rxr, Rxi;
+                load_R(Sx, nuUyy, zc.dPdxRef, Rxr, Rxi);
+                solve_uw(kx2pi, Rxr, Rxi, zc.umean, zc.dPdx_out, Sx);
+            }
+            WV<K> Rzr, Rzi;
+            load_R(Sz, nuWyy, zc.dPdzRef, Rzr, Rzi);
+            solve_uw(kz2pi, Rzr, Rzi, zc.wmean, zc.dPdz_out, Sz);
         } else {  // u and w, real and imaginary parts: four right-hand sides of one operator
             WV<K> f4[4], x4[4];
-            for (int k = 0; k < K; ++k) {
-                f4[0].e[k] = -(kx2pi * Pi.e[k]) - Rxr.e[k];
-                f4[0].o[k] = -(kx2pi * Pi.o[k]) - Rxr.o[k];
-                f4[1].e[k] = kx2pi * Pr.e[k] - Rxi.e[k];
-                f4[1].o[k] = kx2pi * Pr.o[k] - Rxi.o[k];
-                f4[2].e[k] = -(kz2pi * Pi.e[k]) - Rzr.e[k];
-                f4[2].o[k] = -(kz2pi * Pi.o[k]) - Rzr.o[k];
-                f4[3].e[k] = kz2pi * Pr.e[k] - Rzi.e[k];
-                f4[3].o[k] = kz2pi * Pr.o[k] - Rzi.o[k];
+            {
+                WV<K> Rxr, Rxi, Rzr, Rzi;
+                wload_c<K, W>(Sx, c, Rxr, Rxi);
+                wload_c<K, W>(Sz, c, Rzr, Rzi);
+                for (int k = 0; k < K; ++k) {
+                    f4[0].e[k] = -(kx2pi * Pi.e[k]) - Rxr.e[k];
+                    f4[0].o[k] = -(kx2pi * Pi.o[k]) - Rxr.o[k];
+                    f4[1].e[k] = kx2pi * Pr.e[k] - Rxi.e[k];
+                    f4[1].o[k] = kx2pi * Pr.o[k] - Rxi.o[k];
+                    f4[2].e[k] = -(kz2pi * Pi.e[k]) - Rzr.e[k];
+                    f4[2].o[k] = -(kz2pi * Pi.o[k]) - Rzr.o[k];
+                    f4[3].e[k] = kz2pi * Pr.e[k] - Rzi.e[k];
+                    f4[3].o[k] = kz2pi * Pr.o[k] - Rzi.o[k];
+                }
             }
             const double z4[4] = {0.0, 0.0, 0.0, 0.0};
             whelm_n<K, W, 4>(f4, vb, vu, vd, lambda, z4, z4, c, x4);
-            wstore_c<K, W>(lds + (w * 3 + 0) * Ny, c, x4[0], x4[1]);
-            wstore_c<K, W>(lds + (w * 3 + 1) * Ny, c, x4[2], x4[3]);
+            wstore_c<K, W>(Sx, c, x4[0], x4[1]);
+            wstore_c<K, W>(Sz, c, x4[2], x4[3]);
         }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < 2 * Ny * MPB; idx += nthr) {
-        const int ww = idx % MPB, rest = idx / MPB, n = rest % Ny, slot = rest / Ny;
-        const int mm = m0 + ww;
-        if (mm < g.M) {
-            const double2 v = lds[(ww * 3 + slot) * Ny + n];
-            u[((long)(slot == 0 ? 0 : 2) * Ny + n) * g.M + mm] = dvec2{v.x, v.y};
-        }
-    }
+    drain(0, u);                     // u
+    drain(2, u + 2L * Ny * g.M);     // w
 }
 
 // PressureSolver::solve (poissonsolver.cpp:326-431) per mode, after the

@@ -171,7 +171,7 @@ struct NLMember {
 };
 
 // ---- tau ------------------------------------------------------------------
-void launch_tau_setup(const Geom& g, const TauSet& s, double nu, bool taucorr, double* scratch, hipStream_t st);
+void launch_tau_setup(const Geom& g, const TauSet& s, double nu, bool taucorr, hipStream_t st);
 // PressureSolver::solve (poissonsolver.cpp:326-431) of one field: q from the
 // nonlinear term f of u + U and from v; the pressure factors come from set.
 struct PressArgs {

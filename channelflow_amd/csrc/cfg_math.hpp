@@ -19,7 +19,9 @@
 
 // The recurrences below are evaluated without fused multiply-add contraction,
 // so each operation rounds as in the reference's x86-64 build: the tau-
-// corrected pressure amplifies per-operation rounding differences.
+// corrected pressure amplifies per-operation rounding differences. The pragma
+// is lexical and ends with this header (contraction is on again below it), so
+// the kernels that include it keep their fused multiply-adds.
 #pragma clang fp contract(off)
 
 namespace cfg {
@@ -387,3 +389,5 @@ CFG_HD void tau_solve_bulk(const TauView& t, CV u, CV v, CV w, CV P, CV Rx, CV R
 }
 
 }  // namespace cfg
+
+#pragma clang fp contract(fast)

@@ -426,10 +426,30 @@ Exec::~Exec() {
     }
 }
 
+size_t Exec::work_bytes(const Geom& g, int cap) {
+    if (cap <= 0) return 0;
+    return 2 * device_bytes_taken((size_t)cap * nlwork_y_doubles(g) * sizeof(double)) +
+           2 * device_bytes_taken((size_t)cap * nlwork_x_doubles(g) * sizeof(double));
+}
+
+size_t Exec::held_bytes() const { return pool.held() + work_bytes(geo->g, cap_); }
+
+void Exec::trim() {
+    HIPCHK(hipStreamSynchronize(st));
+    drop_graphs();
+    pool.trim();
+    yp_.alloc(0);
+    fo_.alloc(0);
+    xp_.alloc(0);
+    zo_.alloc(0);
+    cap_ = 0;
+    ++buf_gen_;
+}
+
 void Exec::reserve(int n) {
     if (n <= cap_) return;
     HIPCHK(hipStreamSynchronize(st));  // the old work buffers may still be in use
-    const int c = std::max(n, 2 * cap_);
+    const int c = grown_cap(cap_, n);
     const Geom& g = geo->g;
     yp_.alloc((size_t)c * nlwork_y_doubles(g));
     fo_.alloc((size_t)c * nlwork_y_doubles(g));

@@ -75,6 +75,10 @@ class NSECtx {
     FlowModel& model() { return *model_; }
     const cfg_flags& flags() const { return flags_; }
 
+    // the tau sets this NSE holds (device memory accounting)
+    void tau_sets(std::unordered_set<const TauStore*>& out) const {
+        for (auto& t : tau_) out.insert(t.get());
+    }
    private:
     void fetch_dP();
     Exec& ex_;
@@ -132,6 +136,10 @@ class Integrator {
     const FlowModel& model() const { return *model_; }
     const cfg_flags& flags() const { return flags_; }
     Exec& exec() { return ex_; }
+    void tau_sets(std::unordered_set<const TauStore*>& out) const {
+        if (main_nse_) main_nse_->tau_sets(out);
+        if (init_nse_) init_nse_->tau_sets(out);
+    }
 
    private:
     void own_state(TickOps* ops);  // make cur_ the integration's own buffers
@@ -195,6 +203,17 @@ class Exec {
     void append_key(std::string& key) const;
     unsigned long generation() const { return buf_gen_ * 1000003UL + cache.generation(); }
     void issue();  // launches recs_ on st
+
+    // Device memory: the work buffers hold cap() members; a tick of n members
+    // grows them to grown_cap(cap(), n). held_bytes() is what this Exec holds
+    // for its integrations (pool blocks in use or free, work buffers; not its
+    // launch-table arena); trim() hands the pool's free blocks and the work
+    // buffers back to HIP.
+    int cap() const { return cap_; }
+    static int grown_cap(int cap, int n) { return n <= cap ? cap : std::max(n, 2 * cap); }
+    static size_t work_bytes(const Geom& g, int cap);
+    size_t held_bytes() const;
+    void trim();
 
     std::shared_ptr<Geometry> geo;
     hipStream_t st = nullptr;

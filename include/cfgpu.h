@@ -241,7 +241,9 @@ int cfg_batch_tick_stats(const cfg_batch* b, long long* ticks, long long* graph_
  * Continuous batching of integration requests (no counterpart in the
  * reference). A request is (grid, flags, initial u and q, horizon in steps).
  * Queued requests are admitted in order while their device memory fits the
- * budget; every admitted request advances one tick (one nonlinear evaluation)
+ * budget; a request that does not fit yet is passed by later ones that do
+ * (backfilling) until it has waited 256 server steps, after which nothing
+ * passes it; every admitted request advances one tick (one nonlinear evaluation)
  * per cfg_server_step together with the others; a request that reaches its
  * horizon is retired at once and its result is copied back without stalling
  * the rest. */
@@ -258,6 +260,10 @@ int cfg_server_step(cfg_server* s, long long* done, int max_done, int* ndone, in
  * Returns CFG_ERR_NONFINITE when that velocity holds a NaN or an infinity
  * (channelflow/cfdsi.cpp:769-774); the outputs are written all the same. */
 int cfg_server_result(cfg_server* s, long long request, double* u_spec, double* q_spec, double* t);
+/* Device memory the server holds for its integrations, which admission keeps
+ * within the budget: pool blocks (in use or free, at hipMalloc's rounding),
+ * work buffers and tau data. A grid class's fixed tables (launch-table arena,
+ * geometry) are outside the budget; a class that empties hands its memory back. */
 int cfg_server_memory(const cfg_server* s, double* used_bytes);
 
 #ifdef __cplusplus

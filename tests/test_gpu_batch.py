@@ -203,3 +203,71 @@ def test_table_arena_recycle_mid_plan(gpu, monkeypatch):
     print("graph captures/replays: 4 MiB arena", st_ref, "4 KiB arena", st_small, flush=True)
     for (ur, qr), (us, qs) in zip(ref, small):
         assert np.array_equal(ur, us) and np.array_equal(qr, qs)
+
+
+def _device_free_bytes():
+    """hipMemGetInfo through the HIP runtime the library already loaded."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_server_budget_covers_device_memory(gpu):
+    """The memory the server counts against its budget (pool blocks at
+    hipMalloc's rounding, work buffers, tau sets) grows by at least the device
+    memory its integrations really take: state, histories, start-up buffers,
+    work buffers and tau data (a distinct nu per request, so no tau set is
+    shared). Measured with hipMemGetInfo around the admission and first tick of
+    six requests on one grid class that already exists."""
+    grid = (32, 33, 32)
+    Nx, Ny, Nz = grid
+    srv = cfa.Server(budget_bytes=64e9)
+
+    def submit(k):
+        fl = dict(COUETTE, nu=1.0 / (400 + 7 * k))
+        u0 = random_field(Nx, Ny, Nz, LX_C, LZ_C, norm=0.05, decay=0.4, seed=300 + k)
+        srv.submit(fields(grid, u0), cfa.DNSFlags(**fl), 50)
+
+    submit(0)
+    srv.step()  # the class exists: geometry, tables, first work buffers
+    used0, free0 = srv.memory(), _device_free_bytes()
+    for k in range(1, 7):
+        submit(k)
+    srv.step()  # admits the six and runs their first tick (start-up buffers, tau data)
+    used1, free1 = srv.memory(), _device_free_bytes()
+    estimate, actual = used1 - used0, free0 - free1
+    print(f"six 32^3 requests: memory counted {estimate / 1e6:.1f} MB, device memory taken {actual / 1e6:.1f} MB",
+          flush=True)
+    assert actual > 0 and actual <= estimate + (2 << 20)  # hipMemGetInfo moves in 2 MiB steps
+
+
+def test_server_backfills_past_a_large_request(gpu):
+    """A request that does not fit the free budget yet is passed by smaller
+    ones that do (backfilling), and is admitted once memory frees up."""
+    small, big = (16, 17, 16), (24, 25, 24)
+    # admission needs (pool blocks at hipMalloc's rounding, 4 tau sets, work
+    # buffer growth): a small request 1.7 MB (2.5 MB for the third of its
+    # class), the large one 6.3 MB
+    srv = cfa.Server(budget_bytes=7.0e6)
+    ids = []
+    for i, g in enumerate([small, big, small, small]):
+        fl = dict(COUETTE)
+        u0 = random_field(*g, LX_C, LZ_C, norm=0.05, decay=0.4, seed=400 + i)
+        ids.append(srv.submit(fields(g, u0), cfa.DNSFlags(**fl), 6 if g == small else 4))
+    order, running_max, held_max = [], 0, 0.0
+    for _ in range(1000):
+        done, running, queued = srv.step()
+        running_max = max(running_max, running)
+        held_max = max(held_max, srv.memory())
+        for r in done:
+            srv.result(r)
+            order.append(r)
+        if len(order) == 4:
+            break
+    print(f"completion order {order} (submitted {ids}), most running at once {running_max}, "
+          f"most memory held {held_max / 1e6:.2f} MB", flush=True)
+    assert len(order) == 4
+    assert order.index(ids[2]) < order.index(ids[1])  # a small request passed the large one
+    assert held_max <= 7.0e6
