@@ -19,10 +19,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "cfg_kernels.hpp"
 #include "cfg_math.hpp"
@@ -72,17 +75,18 @@ __device__ __forceinline__ void dft_small(double2 (&v)[4]) {
         const double s3 = (SIGN < 0) ? -0.86602540378443864676 : 0.86602540378443864676;
         const double2 s12 = cadd(v[1], v[2]), d12 = csub(v[1], v[2]);
         const double2 m = make_double2(v[0].x - 0.5 * s12.x, v[0].y - 0.5 * s12.y);
-        const double2 q = make_double2(-s3 * d12.y, s3 * d12.x);
         o[0] = cadd(v[0], s12);
-        o[1] = cadd(m, q);
-        o[2] = csub(m, q);
+        o[1] = make_double2(fma(-s3, d12.y, m.x), fma(s3, d12.x, m.y));  // m + q, q = i s3 d12
+        o[2] = make_double2(fma(s3, d12.y, m.x), fma(-s3, d12.x, m.y));  // m - q
     }
     for (int r = 0; r < R; ++r) v[r] = o[r];
 }
 
 // radix 8, decimation in frequency: sums and differences of v_k, v_{k+4}, the
 // differences turned by w^k (w = exp(SIGN 2 pi i/8)), then two radix-4 DFTs
-// give the even and the odd outputs
+// give the even and the odd outputs. The 1/sqrt(2) of w and w^3 is applied in
+// the odd radix-4's first sums, as fused multiply-adds: with p1 = w b1 sqrt2
+// and p3 = w^3 b3 sqrt2, b1 + b3 = h (p1 + p3) and b1 - b3 = h (p1 - p3).
 template <int R, int SIGN>
 __device__ __forceinline__ void dft_small(double2 (&v)[8]) {
     static_assert(R == 8, "radix-8 overload");
@@ -93,17 +97,25 @@ __device__ __forceinline__ void dft_small(double2 (&v)[8]) {
         a[k] = cadd(v[k], v[k + 4]);
         b[k] = csub(v[k], v[k + 4]);
     }
+    double2 p1, p3;
     if constexpr (SIGN < 0) {  // w = (1 - i)/sqrt2, w^2 = -i, w^3 = -(1 + i)/sqrt2
-        b[1] = make_double2(h * (b[1].x + b[1].y), h * (b[1].y - b[1].x));
+        p1 = make_double2(b[1].x + b[1].y, b[1].y - b[1].x);
+        p3 = make_double2(b[3].y - b[3].x, -(b[3].x + b[3].y));
         b[2] = make_double2(b[2].y, -b[2].x);
-        b[3] = make_double2(h * (b[3].y - b[3].x), -h * (b[3].x + b[3].y));
     } else {  // w = (1 + i)/sqrt2, w^2 = i, w^3 = (-1 + i)/sqrt2
-        b[1] = make_double2(h * (b[1].x - b[1].y), h * (b[1].x + b[1].y));
+        p1 = make_double2(b[1].x - b[1].y, b[1].x + b[1].y);
+        p3 = make_double2(-(b[3].x + b[3].y), b[3].x - b[3].y);
         b[2] = make_double2(-b[2].y, b[2].x);
-        b[3] = make_double2(-h * (b[3].x + b[3].y), h * (b[3].x - b[3].y));
     }
     dft_small<4, SIGN>(a);
-    dft_small<4, SIGN>(b);
+    // radix 4 of (b0, h p1, b2, h p3), as in dft_small<4>
+    const double2 s02 = cadd(b[0], b[2]), d02 = csub(b[0], b[2]);
+    const double2 s13 = cadd(p1, p3), d13 = csub(p1, p3);                                // over h
+    const double2 jd = (SIGN < 0) ? make_double2(d13.y, -d13.x) : make_double2(-d13.y, d13.x);  // over h
+    b[0] = make_double2(fma(h, s13.x, s02.x), fma(h, s13.y, s02.y));
+    b[1] = make_double2(fma(h, jd.x, d02.x), fma(h, jd.y, d02.y));
+    b[2] = make_double2(fma(-h, s13.x, s02.x), fma(-h, s13.y, s02.y));
+    b[3] = make_double2(fma(-h, jd.x, d02.x), fma(-h, jd.y, d02.y));
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         v[2 * m] = a[m];
@@ -304,13 +316,15 @@ __device__ TauView make_view(const Geom& g, const TauSet& s, int m, double nu, b
 }
 
 // TauSolver constructor per mode (tausolver.cpp:81-176), T modes per block,
-// one thread each, with the mode's working set in LDS: its 12 tau profiles
-// (array a at element (a*Ny + n)*T + t), 6 scalars and 2 scratch profiles.
-// The arithmetic is tau_setup's, unchanged (the sequential set-up, bit for bit
-// the oracle's); only the memory it runs in moves from HBM to LDS: the
-// recurrences store and reload their profiles, which is an LDS round trip
-// here instead of an HBM one. The profiles then go to the mode-major tau set
-// with contiguous writes.
+// one thread each. ALL: the mode's whole working set in LDS, its 12 tau
+// profiles (array a at element (a*Ny + n)*T + t), 6 scalars and 2 scratch
+// profiles, copied to the mode-major tau set with contiguous writes at the end.
+// Otherwise (tall grids, where a block of 64 KB would hold fewer than 8 such
+// working sets) the profiles and scalars are built in place in the tau set and
+// only the 2 scratch profiles are in LDS. The arithmetic is tau_setup's either
+// way (the sequential set-up, bit for bit the oracle's); only the memory the
+// recurrences store to and reload from moves.
+template <bool ALL>
 __global__ void k_tau_setup(Geom g, TauSet s, double nu, int taucorr) {
     extern __shared__ double sl[];
     const int T = blockDim.x, t = threadIdx.x;
@@ -320,44 +334,60 @@ __global__ void k_tau_setup(Geom g, TauSet s, double nu, int taucorr) {
     if (m < g.M) {
         TauView v = make_view(g, s, m, nu, taucorr != 0);
         auto A = [&](int a) { return SV{sl + a * arr + t, T}; };
-        v.ph.band = A(0);
-        v.ph.up = A(1);
-        v.ph.invd = A(2);
-        v.vh.band = A(3);
-        v.vh.up = A(4);
-        v.vh.invd = A(5);
-        v.Pp = A(6);
-        v.Pm = A(7);
-        v.vp = A(8);
-        v.vm = A(9);
-        v.P0 = A(10);
-        v.v0 = A(11);
-        v.scal = sl + 14 * arr + t;
-        v.sstride = T;
-        tau_setup(v, A(12), A(13));
+        if constexpr (ALL) {
+            v.ph.band = A(0);
+            v.ph.up = A(1);
+            v.ph.invd = A(2);
+            v.vh.band = A(3);
+            v.vh.up = A(4);
+            v.vh.invd = A(5);
+            v.Pp = A(6);
+            v.Pm = A(7);
+            v.vp = A(8);
+            v.vm = A(9);
+            v.P0 = A(10);
+            v.v0 = A(11);
+            v.scal = sl + 14 * arr + t;
+            v.sstride = T;
+            tau_setup(v, A(12), A(13));
+        } else {
+            tau_setup(v, A(0), A(1));
+        }
     }
-    __syncthreads();
-    const int nm = min(T, g.M - m0);
-    __attribute__((address_space(1))) double* coef = gmem_w(s.coef) + (long)m0 * TAU_NA * Ny;
-    for (int i = t; i < nm * TAU_NA * Ny; i += T) {
-        const int mm = i / (TAU_NA * Ny), r = i - mm * (TAU_NA * Ny), a = r / Ny, n = r - a * Ny;
-        coef[i] = sl[a * arr + (long)n * T + mm];
+    if constexpr (ALL) {
+        __syncthreads();
+        const int nm = min(T, g.M - m0);
+        __attribute__((address_space(1))) double* coef = gmem_w(s.coef) + (long)m0 * TAU_NA * Ny;
+        for (int i = t; i < nm * TAU_NA * Ny; i += T) {
+            const int mm = i / (TAU_NA * Ny), r = i - mm * (TAU_NA * Ny), a = r / Ny, n = r - a * Ny;
+            coef[i] = sl[a * arr + (long)n * T + mm];
+        }
+        __attribute__((address_space(1))) double* scal = gmem_w(s.scal) + (long)m0 * 6;
+        for (int i = t; i < nm * 6; i += T) scal[i] = sl[14 * arr + (long)(i % 6) * T + i / 6];
     }
-    __attribute__((address_space(1))) double* scal = gmem_w(s.scal) + (long)m0 * 6;
-    for (int i = t; i < nm * 6; i += T) scal[i] = sl[14 * arr + (long)(i % 6) * T + i / 6];
 }
 
-// modes per block of the set-up: as many as 64 KB of LDS hold (14 profiles
-// and 6 scalars each), at most 64
-static int tau_setup_modes(int Ny) {
-    const long per = (14L * Ny + 6) * (long)sizeof(double);
-    return (int)std::max(1L, std::min(64L, (64L * 1024) / per));
+// modes per block of the set-up within 64 KB of LDS, at most 64: whole working
+// sets (14 profiles and 6 scalars each) where 8 or more fit, else scratch
+// profiles only (2 each)
+static int tau_setup_modes(int Ny, bool* all) {
+    const long whole = (14L * Ny + 6) * (long)sizeof(double), scratch = 2L * Ny * (long)sizeof(double);
+    const long T = (64L * 1024) / whole;
+    *all = T >= 8;
+    return (int)std::max(1L, std::min(64L, (64L * 1024) / (*all ? whole : scratch)));
 }
 
 void launch_tau_setup(const Geom& g, const TauSet& s, double nu, bool taucorr, hipStream_t st) {
-    const int T = tau_setup_modes(g.Ny);
-    const size_t lds = (size_t)(14L * g.Ny + 6) * T * sizeof(double);
-    k_tau_setup<<<(g.M + T - 1) / T, T, lds, st>>>(g, s, nu, taucorr ? 1 : 0);
+    bool all = false;
+    const int T = tau_setup_modes(g.Ny, &all);
+    const dim3 grid((g.M + T - 1) / T);
+    if (all) {
+        const size_t lds = (size_t)(14L * g.Ny + 6) * T * sizeof(double);
+        k_tau_setup<true><<<grid, T, lds, st>>>(g, s, nu, taucorr ? 1 : 0);
+    } else {
+        const size_t lds = (size_t)2 * g.Ny * T * sizeof(double);
+        k_tau_setup<false><<<grid, T, lds, st>>>(g, s, nu, taucorr ? 1 : 0);
+    }
 }
 
 __device__ __forceinline__ CV cview(double* base, const Geom& g, int comp, int m) {
@@ -1494,7 +1524,30 @@ struct ZWCtx {
     double dk;         // 2 pi / Lz
     double2* T;        // the line's LDS rows
     const double2* tw; // exp(-2 pi i m / N), m < N, in LDS
+    // input staging: the wavefront's lines are one contiguous run of the
+    // [field][y][x][kz] arrays (wcount values from wbase); two fields of it sit
+    // in LDS (slots 0 and 1 from S, slot stride sstride), this line's bins at
+    // L * nkz within a slot
+    long wbase;
+    int wcount, lane, sline, sstride;
+    double2* S;
 };
+
+// Starts the copy of field f of the wavefront's lines into staging slot
+// `slot` (global_load_lds_dwordx4: lane i of a copy instruction moves 16 bytes
+// to its base + 16 i; lanes past the run copy nothing). The data are in LDS
+// once the wavefront's vector-memory count has drained; the compiler waits for
+// it before the first LDS read of the staging array.
+template <int NZ>
+__device__ __forceinline__ void zw_stage(const ZWCtx& c, const double2* __restrict__ xp, int f, int slot) {
+    const __attribute__((address_space(1))) double2* src = gmem(xp) + (long)f * c.fsz + c.wbase;
+    double2* dst = c.S + slot * c.sstride;
+    for (int t = 0; t < c.wcount; t += 64) {
+        if (t + c.lane < c.wcount)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + t + c.lane),
+                                             (__attribute__((address_space(3))) void*)(dst + t), 16, 0, 0);
+    }
+}
 
 // DFT (sign SIGN) of a line held in layout A, v[r] = Z[j + N1 r] on lane
 // j < N1, into layout B, u[m] = X[j + N2 m] on lane j < N2:
@@ -1551,13 +1604,19 @@ __device__ __forceinline__ void zw_b2a(double2 (&u)[ZW<N>::N1], double2 (&v)[ZW<
 
 // c2r of two real fields at once (fa -> pa, fb -> pb; fb < 0: none), from the
 // live bins kz < nkz of xp; dza / dzb apply d/dz. As in FFTW's c2r, the
-// imaginary part of bin 0 is ignored.
-template <int NZ>
+// imaginary part of bin 0 is ignored. STAGED: the bins come from staging
+// slots 0 (fa) and 1 (fb), and once these are read the copies of fields na and
+// nb (< 0: none) into them start, to land while this transform and what
+// follows it run.
+template <int NZ, bool STAGED>
 __device__ __forceinline__ void zw_c2r(const ZWCtx& c, const double2* __restrict__ xp, int fa, bool dza, int fb,
-                                       bool dzb, double (&pa)[ZW<NZ>::N1], double (&pb)[ZW<NZ>::N1]) {
+                                       bool dzb, double (&pa)[ZW<NZ>::N1], double (&pb)[ZW<NZ>::N1], int na = -1,
+                                       int nb = -1) {
     constexpr int N1 = ZW<NZ>::N1, N2 = ZW<NZ>::N2, N = ZW<NZ>::N;
     const int j = c.gi;
     const bool act = c.line_ok && j < N1;
+    const double2* Sa = c.S + c.sline;
+    const double2* Sb = Sa + c.sstride;
     double2 v[N2];
 #pragma unroll
     for (int r = 0; r < N2; ++r) {
@@ -1566,14 +1625,25 @@ __device__ __forceinline__ void zw_c2r(const ZWCtx& c, const double2* __restrict
         const int k = lo ? n : N - n;
         double2 A = make_double2(0.0, 0.0), B = make_double2(0.0, 0.0);
         if (act && (lo || hi)) {
-            A = xp[(long)fa * c.fsz + c.base + k];
-            if (fb >= 0) B = xp[(long)fb * c.fsz + c.base + k];
+            if constexpr (STAGED) {
+                A = Sa[k];
+                if (fb >= 0) B = Sb[k];
+            } else {
+                A = xp[(long)fa * c.fsz + c.base + k];
+                if (fb >= 0) B = xp[(long)fb * c.fsz + c.base + k];
+            }
         }
         if (dza) A = imul(c.dk * k, A);
         if (dzb) B = imul(c.dk * k, B);
         if (n == 0) v[r] = make_double2(A.x, B.x);
         else if (lo) v[r] = make_double2(A.x - B.y, A.y + B.x);  // A + iB
         else v[r] = make_double2(A.x + B.y, B.x - A.y);          // conj(A) + i conj(B) (zero in the gap)
+    }
+    if (STAGED && na >= 0) {
+        // every lane's reads of the slots have returned before the copies may overwrite them
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        zw_stage<NZ>(c, xp, na, 0);
+        if (nb >= 0) zw_stage<NZ>(c, xp, nb, 1);
     }
     double2 u[N1];
     zw_a2b<NZ, +1>(v, u, j, c.T, c.tw);
@@ -1624,24 +1694,57 @@ __device__ __forceinline__ void zw_r2c(const ZWCtx& c, double2* __restrict__ zo,
     }
 }
 
+// wavefronts per block of the staged wavefront z pass
+constexpr int ZWAVE_WB = 2;
+
+// The z pass stages its inputs at Nz = 64 only. The choice depends on the grid
+// alone (never on the member count), so a member's result is the same alone
+// and in a batch: the two forms contract multiply-adds differently (the
+// compiler fuses either product of a*b + c*d depending on the code around
+// it) and differ in the last bits. At Nz = 64 the staged form wins where it
+// matters, in batches (16 members of 64x65x64, skew-symmetric: 104 against
+// 122 us per launch) and loses 2.6 us on a lone field; at 32 and 48 (24
+// members) and for lone 128^3 and 192^3 fields the unstaged one is faster.
+constexpr bool zwave_staged(int nz) { return nz == 64; }
+
 // The z pass of k_zpass with one wavefront per 64/G lines (same fields, same
-// products in the same order). A block is 1, 2 or 4 independent wavefronts.
-template <int METHOD, int NZ>
+// products in the same order). A block is independent wavefronts: any number
+// up to 4 (LDS sized at launch), or, STAGED, ZWAVE_WB of them, each staging
+// the next transform's two input fields in LDS while it works on the current
+// one. Staging hides the loads' latency where many wavefronts share a CU;
+// with few wavefronts per SIMD the extra LDS round trip costs more than it
+// hides (zwave_staged picks the form per grid).
+template <int METHOD, int NZ, bool STAGED>
 __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict__ twg, const NLMember* __restrict__ tab,
                                                int want_g, const double2* __restrict__ xp0, double2* __restrict__ zo0,
                                                long sx) {
     using S = ZW<NZ>;
-    constexpr int N1 = S::N1;
-    extern __shared__ double2 lds[];
-    for (int i = threadIdx.x; i < NZ; i += blockDim.x) lds[i] = twg[i];
+    constexpr int N1 = S::N1, SLOT = S::LPW * (NZ / 2);
+    double2 *tw_l, *T_l, *in_l = nullptr;
+    if constexpr (STAGED) {
+        // Separate LDS arrays (not one dynamic block) let the compiler tell the
+        // staging copies apart from the transposes: an LDS read waits for the
+        // copies in flight only when it reads the staging array.
+        __shared__ double2 tw_s[NZ];
+        __shared__ double2 T_s[ZWAVE_WB * S::LPW * S::LBUF];
+        __shared__ double2 in_s[ZWAVE_WB * 2 * SLOT];
+        tw_l = tw_s;
+        T_l = T_s;
+        in_l = in_s;
+    } else {
+        extern __shared__ double2 lds[];
+        tw_l = lds;
+        T_l = lds + NZ;
+    }
+    for (int i = threadIdx.x; i < NZ; i += blockDim.x) tw_l[i] = twg[i];
     __syncthreads();
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), lane = threadIdx.x % 64;
     const int nxc = (g.Nx + S::LPW - 1) / S::LPW;
     const int wg = blockIdx.x * (blockDim.x / 64) + wave;
     const int y = wg / nxc;
     if (y >= g.Ny) return;
     const int L = lane / S::G;
-    const int x = (wg - y * nxc) * S::LPW + L;
+    const int x0 = (wg - y * nxc) * S::LPW, x = x0 + L;
     ZWCtx c;
     c.gi = lane % S::G;
     c.line_ok = x < g.Nx;
@@ -1649,16 +1752,26 @@ __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict
     c.fsz = (long)g.Ny * g.Nx * g.nkz;
     c.nkz = g.nkz;
     c.dk = 2 * PI / g.Lz;
-    c.T = lds + NZ + (wave * S::LPW + L) * S::LBUF;
-    c.tw = lds;
+    c.T = T_l + (wave * S::LPW + L) * S::LBUF;
+    c.tw = tw_l;
+    c.wbase = ((long)y * g.Nx + x0) * g.nkz;
+    c.wcount = min(S::LPW, g.Nx - x0) * g.nkz;
+    c.lane = lane;
+    c.sline = L * g.nkz;
+    c.sstride = SLOT;
+    c.S = in_l + wave * 2 * SLOT;
     const double rotation = tab[blockIdx.y].rotation;
     const double2* __restrict__ xp = xp0 + blockIdx.y * sx;
     double2* __restrict__ zo = zo0 + blockIdx.y * sx;
+    if constexpr (STAGED) {
+        zw_stage<NZ>(c, xp, 0, 0);  // u, v: the first transform's pair
+        zw_stage<NZ>(c, xp, 1, 1);
+    }
     if constexpr (METHOD == NL_ROT) {
         double u0[N1], u1[N1], u2[N1], w0[N1], w1[N1], w2[N1];
-        zw_c2r<NZ>(c, xp, 0, false, 1, false, u0, u1);
-        zw_c2r<NZ>(c, xp, 2, false, 3, false, u2, w0);
-        zw_c2r<NZ>(c, xp, 4, false, 5, false, w1, w2);
+        zw_c2r<NZ, STAGED>(c, xp, 0, false, 1, false, u0, u1, 2, 3);
+        zw_c2r<NZ, STAGED>(c, xp, 2, false, 3, false, u2, w0, 4, 5);
+        zw_c2r<NZ, STAGED>(c, xp, 4, false, 5, false, w1, w2);
         double f0[N1], f1[N1], f2[N1];
 #pragma unroll
         for (int m = 0; m < N1; ++m) {  // cross(vort, u) (diffops.cpp:2598-2605)
@@ -1688,8 +1801,8 @@ __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict
         constexpr bool SK = (METHOD == NL_SKEW);
         constexpr double s = SK ? 0.5 : 1.0, us = SK ? 2.0 : 1.0;
         double h0[N1], h1[N1], h2[N1], pa[N1], pb[N1], fa[N1], fb[N1];
-        zw_c2r<NZ>(c, xp, 0, false, 1, false, h0, h1);   // u, v
-        zw_c2r<NZ>(c, xp, 2, false, 6, false, h2, pa);   // w, du/dx
+        zw_c2r<NZ, STAGED>(c, xp, 0, false, 1, false, h0, h1, 2, 6);   // u, v
+        zw_c2r<NZ, STAGED>(c, xp, 2, false, 6, false, h2, pa, 3, 0);   // w, du/dx
         if constexpr (SK) {
 #pragma unroll
             for (int m = 0; m < N1; ++m) {
@@ -1700,7 +1813,7 @@ __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict
         }
 #pragma unroll
         for (int m = 0; m < N1; ++m) fa[m] = 0.0 + h0[m] * pa[m];
-        zw_c2r<NZ>(c, xp, 3, false, 0, true, pa, pb);    // du/dy, du/dz
+        zw_c2r<NZ, STAGED>(c, xp, 3, false, 0, true, pa, pb, 7, 4);    // du/dy, du/dz
 #pragma unroll
         for (int m = 0; m < N1; ++m) {
             fa[m] += h1[m] * pa[m];
@@ -1710,20 +1823,20 @@ __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict
         }
         if constexpr (SK) zw_r2c<NZ, true>(c, zo, fa, pb, 0, 5, 0.5);  // f0 + i kz u0u2 / 2, and u0u2
         else zw_r2c<NZ, false>(c, zo, fa, pb, 0, -1);                  // f0
-        zw_c2r<NZ>(c, xp, 7, false, 4, false, pa, pb);   // dv/dx, dv/dy
+        zw_c2r<NZ, STAGED>(c, xp, 7, false, 4, false, pa, pb, 1, 8);   // dv/dx, dv/dy
 #pragma unroll
         for (int m = 0; m < N1; ++m) {
             fa[m] = 0.0 + h0[m] * pa[m];
             fa[m] += h1[m] * pb[m];
         }
-        zw_c2r<NZ>(c, xp, 1, true, 8, false, pa, pb);    // dv/dz, dw/dx
+        zw_c2r<NZ, STAGED>(c, xp, 1, true, 8, false, pa, pb, 5, 2);    // dv/dz, dw/dx
 #pragma unroll
         for (int m = 0; m < N1; ++m) {
             fa[m] += h2[m] * pa[m];
             if (rotation != 0.0) fa[m] += rotation * (us * h0[m]);
             fb[m] = 0.0 + h0[m] * pb[m];
         }
-        zw_c2r<NZ>(c, xp, 5, false, 2, true, pa, pb);    // dw/dy, dw/dz
+        zw_c2r<NZ, STAGED>(c, xp, 5, false, 2, true, pa, pb);    // dw/dy, dw/dz
 #pragma unroll
         for (int m = 0; m < N1; ++m) {
             fb[m] += h1[m] * pa[m];
@@ -1751,8 +1864,8 @@ __global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict
         }
     } else {
         double u0[N1], u1[N1], u2[N1], pa[N1], pb[N1];
-        zw_c2r<NZ>(c, xp, 0, false, 1, false, u0, u1);
-        zw_c2r<NZ>(c, xp, 2, false, -1, false, u2, pa);
+        zw_c2r<NZ, STAGED>(c, xp, 0, false, 1, false, u0, u1, 2);
+        zw_c2r<NZ, STAGED>(c, xp, 2, false, -1, false, u2, pa);
         if (want_g) {  // DIV with rotation: g = rotation terms only
 #pragma unroll
             for (int m = 0; m < N1; ++m) {
@@ -2629,12 +2742,18 @@ static void launch_z_t(const Geom& g, const FFTPlan& pz, int n, const NLMember* 
                        double2* zo, long sx, const NLTune& tn, hipStream_t st) {
     if constexpr (zwave_size(NZ)) {
         using S = ZW<NZ>;
-        const int wb = tn.zwb > 0 ? tn.zwb : 4;
         const long waves = (long)g.Ny * ((g.Nx + S::LPW - 1) / S::LPW);
+        if (tn.zstage < 0 ? zwave_staged(NZ) : tn.zstage != 0) {
+            if (g.nkz > NZ / 2) throw std::runtime_error("z pass: more live kz bins than Nz/2");
+            const dim3 grid((unsigned)((waves + ZWAVE_WB - 1) / ZWAVE_WB), n);
+            k_zwave<METHOD, NZ, true><<<grid, 64 * ZWAVE_WB, 0, st>>>(g, pz.tw, tab, want_g, xp, zo, sx);
+            return;
+        }
+        const int wb = tn.zwb > 0 ? tn.zwb : 4;
         const size_t lds = (size_t)(NZ + wb * S::LPW * S::LBUF) * sizeof(double2);
         const dim3 grid((unsigned)((waves + wb - 1) / wb), n);
-        allow_lds(k_zwave<METHOD, NZ>, lds);
-        k_zwave<METHOD, NZ><<<grid, 64 * wb, lds, st>>>(g, pz.tw, tab, want_g, xp, zo, sx);
+        allow_lds(k_zwave<METHOD, NZ, false>, lds);
+        k_zwave<METHOD, NZ, false><<<grid, 64 * wb, lds, st>>>(g, pz.tw, tab, want_g, xp, zo, sx);
         return;
     }
     const int nphys = zpass_nphys(METHOD);
@@ -2784,66 +2903,86 @@ NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
         (void)hipEventDestroy(a);
         return NLTune{};
     }
-    auto timed = [&](auto&& launch) {
-        launch();
-        float best = 1e30f;
-        for (int r = 0; r < 3; ++r) {
-            (void)hipEventRecord(a, st);
-            launch();
-            (void)hipEventRecord(b, st);
-            (void)hipEventSynchronize(b);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, a, b);
-            best = std::min(best, ms);
+    // Every candidate runs once to warm up, then all are timed in 5 interleaved
+    // rounds (so clock ramps and neighbours' load fall on all of them alike);
+    // each keeps its best time.
+    auto pick = [&](const std::vector<NLTune>& cands, auto&& launch, const char* what) {
+        for (const NLTune& c : cands) launch(c);
+        std::vector<float> best(cands.size(), 1e30f);
+        for (int r = 0; r < 5; ++r)
+            for (size_t i = 0; i < cands.size(); ++i) {
+                (void)hipEventRecord(a, st);
+                launch(cands[i]);
+                (void)hipEventRecord(b, st);
+                (void)hipEventSynchronize(b);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, a, b);
+                best[i] = std::min(best[i], ms);
+            }
+        size_t k = 0;
+        for (size_t i = 1; i < cands.size(); ++i)
+            if (best[i] < best[k]) k = i;
+        if (std::getenv("CFGPU_TUNE_LOG")) {
+            std::fprintf(stderr, "cfg tune %s %dx%dx%d form %d members %d:", what, g.Nx, g.Ny, g.Nz, method, n);
+            for (size_t i = 0; i < cands.size(); ++i)
+                std::fprintf(stderr, " [zwb %d zstage %d zxb %d xwb %d xl %d] %.1f us%s", cands[i].zwb,
+                             cands[i].zstage, cands[i].zxb, cands[i].xwb, cands[i].xl, best[i] * 1e3f,
+                             i == k ? " *" : "");
+            std::fprintf(stderr, "\n");
         }
-        return best;
+        return cands[k];
     };
     NLTune out;
-    float bz = 1e30f;
-    if (zwave_size(g.Nz)) {  // wavefront z pass: wavefronts per block
-        for (int wb = 1; wb <= 4; wb *= 2) {
-            NLTune c;
-            c.zwb = wb;
-            const float ms = timed([&] { launch_z_any(method, g, pz, n, tab, want_g, xp, zo, sx, c, st); });
-            if (ms < bz) {
-                bz = ms;
-                out.zwb = wb;
-            }
-        }
-    }
-    const int xb_max = zwave_size(g.Nz) ? 0 : zpass_xb_max(g, nphys);
-    for (int xb = 1; xb <= xb_max; xb *= 2) {
+    std::vector<NLTune> zc;
+    if (zwave_size(g.Nz)) {  // wavefront z pass: the staged form, or wavefronts per block
+        // the grid's form (zwave_staged), or the one CFGPU_ZWAVE_STAGED=1 / 0 names
+        const char* force = std::getenv("CFGPU_ZWAVE_STAGED");
+        const bool staged = force ? force[0] != '0' : zwave_staged(g.Nz), unstaged = !staged;
         NLTune c;
-        c.zxb = xb;
-        const float ms = timed([&] { launch_z_any(method, g, pz, n, tab, want_g, xp, zo, sx, c, st); });
-        if (ms < bz) {
-            bz = ms;
-            out.zxb = xb;
+        if (staged) {
+            c.zwb = ZWAVE_WB;
+            c.zstage = 1;
+            zc.push_back(c);
+        }
+        for (int wb = 1; unstaged && wb <= 4; wb *= 2) {
+            c.zwb = wb;
+            c.zstage = 0;
+            zc.push_back(c);
+        }
+    } else {  // LDS z pass: lines per block
+        for (int xb = 1; xb <= zpass_xb_max(g, nphys); xb *= 2) {
+            NLTune c;
+            c.zxb = xb;
+            zc.push_back(c);
         }
     }
-    float bx = 1e30f;
-    if (zwave_size(g.Nx)) {  // wavefront x-FFTs: wavefronts per block, timed on the inverse
+    if (!zc.empty()) {
+        const NLTune z = pick(zc, [&](const NLTune& c) { launch_z_any(method, g, pz, n, tab, want_g, xp, zo, sx, c, st); },
+                              "z");
+        out.zwb = z.zwb;
+        out.zstage = z.zstage;
+        out.zxb = z.zxb;
+    }
+    std::vector<NLTune> xc;  // x-FFTs, timed on the inverse
+    if (zwave_size(g.Nx)) {  // wavefront x-FFTs: wavefronts per block
         for (int wb = 1; wb <= 4; wb *= 2) {
             NLTune c;
             c.xwb = wb;
-            const float ms = timed([&] { launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, n, yp,
-                                                     sy, xp, sx, c, st); });
-            if (ms < bx) {
-                bx = ms;
-                out.xwb = wb;
-            }
+            xc.push_back(c);
+        }
+    } else {  // LDS x-FFTs: lines per block
+        for (int xl = 1; xl <= xfft_lines_max(g); xl *= 2) {
+            NLTune c;
+            c.xl = xl;
+            xc.push_back(c);
         }
     }
-    const int xl_max = zwave_size(g.Nx) ? 0 : xfft_lines_max(g);
-    for (int xl = 1; xl <= xl_max; xl *= 2) {
-        NLTune c;
-        c.xl = xl;
-        const float ms = timed([&] { launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, n, yp, sy,
-                                                 xp, sx, c, st); });
-        if (ms < bx) {
-            bx = ms;
-            out.xl = xl;
-        }
+    if (!xc.empty()) {
+        const NLTune x = pick(xc, [&](const NLTune& c) {
+            launch_xfft(true, g, px, nsrc == 6 && nxout == 9 ? 6 : nxout, nxout, n, yp, sy, xp, sx, c, st);
+        }, "x");
+        out.xwb = x.xwb;
+        out.xl = x.xl;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

@@ -211,7 +211,8 @@ inline long nlwork_x_doubles(const Geom& g) { return 2L * 9 * g.Ny * g.Nx * g.nk
 struct NLTune {
     int zxb = 0;  // (y, x) lines per block of the LDS z-pass
     int xl = 0;   // kz lines per block of the LDS x-FFTs
-    int zwb = 0;  // wavefronts per block of the wavefront z-pass (0: 4)
+    int zwb = 0;      // wavefronts per block of the unstaged wavefront z-pass (0: 4)
+    int zstage = -1;  // wavefront z-pass stages its inputs in LDS: 1 / 0, or -1 the grid's rule
     int xwb = 0;  // wavefronts per block of the wavefront x-FFTs (0: 4)
 };
 

@@ -205,6 +205,43 @@ def test_table_arena_recycle_mid_plan(gpu, monkeypatch):
         assert np.array_equal(ur, us) and np.array_equal(qr, qs)
 
 
+def test_zpass_staged_and_unstaged_agree(gpu, monkeypatch):
+    """The wavefront z-pass has two forms: one that stages each transform's
+    input fields in LDS ahead of use (LDS DMA copies), the default at Nz = 64,
+    and one that loads them as it goes. Their arithmetic is the same up to
+    which product of a*b + c*d the compiler fuses into a multiply-add, so every
+    member must agree under the two to rounding: after 8 steps 1e-14 of the
+    velocity's size, and 1e-12 of the pressure's, which the tau correction
+    makes sensitive to rounding (a misplaced or stale staged bin would be off
+    by the field's own size, 1e-3 and more). Members cover the four
+    nonlinear forms, the wavefront lengths 16, 32, 48 and 64, and grids without
+    dealiasing (Nz/2 live kz bins, the staging's largest)."""
+    specs = [((16, 17, 16), "SkewSymmetric", "DealiasXZ"), ((32, 17, 32), "Rotational", "DealiasXZ"),
+             ((48, 17, 48), "Convection", "DealiasXZ"), ((64, 17, 64), "SkewSymmetric", "DealiasXZ"),
+             ((32, 17, 32), "Divergence", "NoDealiasing"), ((48, 17, 48), "SkewSymmetric", "NoDealiasing"),
+             ((64, 17, 64), "Rotational", "DealiasXZ"), ((64, 17, 64), "Convection", "NoDealiasing")]
+
+    def run():
+        b = cfa.Batch()
+        ids = []
+        for i, (grid, nl, de) in enumerate(specs):
+            fl = dict(COUETTE, nonlinearity=nl, dealiasing=de)
+            u0 = random_field(*grid, LX_C, LZ_C, norm=0.05 + 0.02 * i, decay=0.4, seed=400 + i)
+            ids.append(b.add(fields(grid, u0), cfa.DNSFlags(**fl)))
+        b.advance(8)
+        return [b.state(k) for k in ids]
+
+    monkeypatch.setenv("CFGPU_ZWAVE_STAGED", "0")
+    loads = run()
+    monkeypatch.setenv("CFGPU_ZWAVE_STAGED", "1")
+    staged = run()
+    for (grid, nl, de), (ul, ql), (us, qs) in zip(specs, loads, staged):
+        du = float(np.max(np.abs(ul - us))) / float(np.max(np.abs(ul)))
+        dq = float(np.max(np.abs(ql - qs))) / float(np.max(np.abs(ql)))
+        print(grid, nl, de, f"max|staged - unstaged| / max|u| {du:.2e}, q {dq:.2e}", flush=True)
+        assert du <= 1e-12 and dq <= 1e-9
+
+
 def _device_free_bytes():
     """hipMemGetInfo through the HIP runtime the library already loaded."""
     import ctypes as C
