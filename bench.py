@@ -191,6 +191,42 @@ def tau_factor_bytes(grid):
 # ---------------------------------------------------------------------------
 # runs
 # ---------------------------------------------------------------------------
+class FiniteAdvance:
+    """batch.advance(n) for an ensemble whose members may stop being finite.
+    Such members are stopped, as the reference's batch client stops them
+    (channelflow/cfdsi.cpp:769-774): removed from the batch and recorded in
+    `dropped` with the step count after which they were found; `ids` (member
+    index -> batch id) keeps the others."""
+
+    def __init__(self, cfa, batch, members):
+        self.cfa, self.batch, self.members = cfa, batch, members
+        self.ids, self.dropped, self.steps = {}, [], 0
+
+    def add(self, i):
+        m = self.members[i]
+        self.ids[i] = self.batch.add(flow_fields(self.cfa, m["grid"], member_field(m)),
+                                     self.cfa.DNSFlags(**m["flags"]))
+
+    def points(self):
+        return sum(np.prod(self.members[i]["grid"]) for i in self.ids)
+
+    def __call__(self, nsteps):
+        try:
+            self.batch.advance(nsteps)
+            bad = []
+        except self.cfa.CfgError as e:
+            if e.code != self.cfa.ERR_NONFINITE:
+                raise
+            bad = [i for i, k in self.ids.items() if not self.batch.finite(k)]
+        self.steps += nsteps
+        for i in bad:
+            self.batch.remove(self.ids.pop(i))
+            m = self.members[i]
+            self.dropped.append(dict(member=i, grid="x".join(map(str, m["grid"])), norm=round(m["norm"], 4),
+                                     nonfinite_after_step=self.steps))
+        return bad
+
+
 def run_batched(cfa, members, steps, warmup, prof_steps):
     """Timed region: `steps` steps of the whole ensemble with each class's
     repeating ticks replayed as hipGraphs and no events between launches.
@@ -200,25 +236,31 @@ def run_batched(cfa, members, steps, warmup, prof_steps):
     launch's events time that kernel, not the other classes' kernels running
     beside it on their streams; the sums over the classes are returned."""
     batch = cfa.Batch()
-    for mem in members:
-        batch.add(flow_fields(cfa, mem["grid"], member_field(mem)), cfa.DNSFlags(**mem["flags"]))
-    batch.advance(warmup)                 # SMRK2 start + warm-up SBDF3 steps (and graph captures), untimed
-    extra = steady_state(batch, 1)
+    adv = FiniteAdvance(cfa, batch, members)
+    for i in range(len(members)):
+        adv.add(i)
+    adv(warmup)                           # SMRK2 start + warm-up SBDF3 steps (and graph captures), untimed
+    extra = steady_state(batch, 1, advance=adv)
     c0, r0 = batch.graph_stats()
     k0, l0, h0 = batch.tick_stats()
     t0 = time.perf_counter()
-    batch.advance(steps)                  # returns after every stream has drained
+    # returns after every stream has drained; a member found not finite at
+    # the end has run every step, but its points are not counted
+    adv(steps)
     wall = time.perf_counter() - t0
+    pts = adv.points()
+    timed = sorted(adv.ids)  # the members counted in the timed region
     c1, r1 = batch.graph_stats()
     k1, l1, h1 = batch.tick_stats()
     # the same steps with one graph per grid class and tick, the classes on
     # their own streams
     batch.set_graphs(cfa.Batch.GRAPHS_PER_CLASS)
-    steady_state(batch, batch.size()[1])
+    steady_state(batch, batch.size()[1], advance=adv)
     k2, l2, h2 = batch.tick_stats()
     t2 = time.perf_counter()
-    batch.advance(steps)
+    adv(steps)
     wall_w = time.perf_counter() - t2
+    pts_w = adv.points()
     k3, l3, h3 = batch.tick_stats()
     batch.set_graphs(True)
     kt = {}
@@ -237,34 +279,34 @@ def run_batched(cfa, members, steps, warmup, prof_steps):
         cls.profile(False)
         del cls
     kt = {k: tuple(v) for k, v in kt.items()}
-    for mid in range(0, len(members), max(1, len(members) // 8)):
-        u, _ = batch.state(mid)
-        if not np.all(np.isfinite(u)):
-            raise SystemExit(f"non-finite state of member {mid}")
-    pts = sum(np.prod(m["grid"]) for m in members)
+    for i, k in adv.ids.items():  # every member still in the batch is finite (advance checked it)
+        if not batch.finite(k):
+            raise SystemExit(f"non-finite state of member {i}")
     return dict(value=pts * steps / wall, wall=wall, kt=kt, pts=pts, classes=batch.size()[1],
+                dropped_nonfinite=adv.dropped, steps_run=adv.steps, timed_members=timed,
                 graphs={"mode": "one graph per tick for the whole batch (grid classes as parallel branches)",
                         "captured_in_timed_region": c1 - c0, "replayed_in_timed_region": r1 - r0,
                         "ticks_in_timed_region": k1 - k0, "graph_launches_per_step": (l1 - l0) / steps,
                         "host_us_per_tick": (h1 - h0) / max(1, k1 - k0), "warmup_steps_added": extra,
                         "per_class": {"mode": "one graph per grid class and tick, the classes on their own "
                                               "streams; same members and steps, after the timed region",
-                                      "value": pts * steps / wall_w, "ms_per_step": wall_w / steps * 1e3,
+                                      "value": pts_w * steps / wall_w, "ms_per_step": wall_w / steps * 1e3,
                                       "graph_launches_per_step": (l3 - l2) / steps,
                                       "host_us_per_tick": (h3 - h2) / max(1, k3 - k2)}})
 
 
-def steady_state(batch, per_tick, period=6, max_extra=60):
+def steady_state(batch, per_tick, period=6, max_extra=60, advance=None):
     """Extends the warm-up until the ensemble's ticks are all graph replays:
     a block of `period` steps (a multiple of the SBDF3 history rotation) that
     captures nothing and replays `per_tick` graphs per tick (the grid classes
-    with one graph per class, 1 with one graph for the whole batch). Returns
-    the steps added."""
+    with one graph per class, 1 with one graph for the whole batch). Steps
+    through `advance` (default batch.advance). Returns the steps added."""
+    advance = advance or batch.advance
     extra = 0
     while extra < max_extra:
         c0, r0 = batch.graph_stats()
         k0 = batch.tick_stats()[0]
-        batch.advance(period)
+        advance(period)
         extra += period
         c1, r1 = batch.graph_stats()
         k1 = batch.tick_stats()[0]
@@ -601,12 +643,17 @@ def main():
         "data": "synthetic (seeded divergence-free perturbations)",
         "config": {"workload": label, "members": len(members), "grid_classes": b["classes"],
                    "grid_points_per_step": int(b["pts"]), "parallelism": "batched members, one GPU"},
+        # members whose velocity stopped being finite (the reference's batch
+        # client stops them too): removed, and not counted from the timed
+        # region they failed in
+        "members_stopped_nonfinite": b["dropped_nonfinite"],
+        "ensemble_steps_run": b["steps_run"],
         "graphs": b["graphs"],
         "roofline": roofline(members, b["kt"], prof_steps),
         "kernels_ms_per_step": {k: round(v[0] / prof_steps, 4) for k, v in b["kt"].items() if v[1] > 0},
         "kernel_profile_steps": prof_steps,
     }
-    B_step = sum(16.0 * 25 * live(m["grid"]) for m in members)
+    B_step = sum(16.0 * 25 * live(members[i]["grid"]) for i in b["timed_members"])
     out["step_roofline"] = {"bytes_per_step": B_step, "achieved": B_step / (b["wall"] / args.steps) / 1e9,
                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": B_step / (b["wall"] / args.steps) / 1e9 / HBM_PEAK_GBS}

@@ -30,7 +30,8 @@ EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_des
             "cfg_dns_cfl", "cfg_dns_base", "cfg_dns_device_state", "cfg_dns_profile", "cfg_dns_kernel_times",
             "cfg_kernel_name", "cfg_nonlinear", "cfg_nonlinear_noalloc", "cfg_solve", "cfg_pressure",
             "cfg_batch_create", "cfg_batch_destroy", "cfg_batch_add", "cfg_batch_remove", "cfg_batch_advance",
-            "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_dpdx",
+            "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_finite",
+            "cfg_batch_dpdx",
             "cfg_batch_cfl", "cfg_batch_reset_dt", "cfg_batch_advance_members", "cfg_batch_profile", "cfg_batch_kernel_times", "cfg_batch_set_graphs", "cfg_batch_graph_stats", "cfg_batch_tick_stats",
             "cfg_server_create", "cfg_server_destroy", "cfg_server_submit", "cfg_server_step", "cfg_server_result",
             "cfg_server_memory", "cfg_tausolver_create", "cfg_tausolver_destroy", "cfg_tausolver_solve",
@@ -109,6 +110,7 @@ def library():
         "cfg_batch_get_state": ([v, i, P, P], i),
         "cfg_batch_set_state": ([v, i, P, P], i),
         "cfg_batch_time": ([v, i, P], i),
+        "cfg_batch_finite": ([v, i, C.POINTER(C.c_int)], i),
         "cfg_batch_dpdx": ([v, i, P], i),
         "cfg_batch_cfl": ([v, i, P], i),
         "cfg_batch_reset_dt": ([v, i, d], i),
@@ -511,6 +513,13 @@ class Batch:
         t = C.c_double()
         _check(library().cfg_batch_time(self._h, int(member), C.byref(t)))
         return t.value
+
+    def finite(self, member):
+        """Whether the member's velocity was finite after its last advance (a
+        CFG_ERR_NONFINITE of advance names the members for which it was not)."""
+        v = C.c_int()
+        _check(library().cfg_batch_finite(self._h, int(member), C.byref(v)))
+        return bool(v.value)
 
     def dPdx(self, member):
         v = C.c_double()

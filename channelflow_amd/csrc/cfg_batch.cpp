@@ -702,6 +702,13 @@ int cfg_batch_time(cfg_batch* b, int member, double* t) {
     });
 }
 
+int cfg_batch_finite(cfg_batch* b, int member, int* finite) {
+    return guarded([&] {
+        if (!b || !finite) throw Error(CFG_ERR_ARG, "null argument");
+        *finite = b->b.get(member).finite ? 1 : 0;
+    });
+}
+
 int cfg_batch_dpdx(cfg_batch* b, int member, double* v) {
     return guarded([&] {
         if (!b || !v) throw Error(CFG_ERR_ARG, "null argument");

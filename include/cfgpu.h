@@ -206,6 +206,11 @@ int cfg_batch_size(const cfg_batch* b, int* nmembers, int* nclasses);
 int cfg_batch_get_state(cfg_batch* b, int member, double* u_spec, double* q_spec);
 int cfg_batch_set_state(cfg_batch* b, int member, const double* u_spec, const double* q_spec);
 int cfg_batch_time(cfg_batch* b, int member, double* t);
+/* Whether the member's velocity was finite after its last advance (1) or held
+ * a NaN or an infinity (0): the members a CFG_ERR_NONFINITE of
+ * cfg_batch_advance names. No counterpart in the reference, whose batch
+ * client tests L2Norm(u) itself (channelflow/cfdsi.cpp:769-774). */
+int cfg_batch_finite(cfg_batch* b, int member, int* finite);
 int cfg_batch_dpdx(cfg_batch* b, int member, double* dPdx);
 /* DNS::CFL(u) of one member's current velocity (dns.cpp:282-289). */
 int cfg_batch_cfl(cfg_batch* b, int member, double* cfl);

@@ -306,6 +306,13 @@ class DNSBatch {
         check(cfg_batch_set_state(h_, id, spectral_data(fields[0]), spectral_data(fields[1])), "cfg_batch_set_state");
     }
     void reset_dt(int id, Real dt) { check(cfg_batch_reset_dt(h_, id, dt), "cfg_batch_reset_dt"); }
+    /* the member's velocity was finite after its last advance (an Error with
+       code CFG_ERR_NONFINITE from advance names the members for which not) */
+    bool finite(int id) {
+        int f = 0;
+        check(cfg_batch_finite(h_, id, &f), "cfg_batch_finite");
+        return f != 0;
+    }
     Real time(int id) {
         double t = 0;
         check(cfg_batch_time(h_, id, &t), "cfg_batch_time");

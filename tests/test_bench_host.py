@@ -119,3 +119,74 @@ def test_steady_state_ends_on_a_replay_only_block(eager, captures, expect):
 def test_steady_state_gives_up():
     with pytest.raises(SystemExit):
         bench.steady_state(_FakeBatch(1000, 0), 1)
+
+
+class _FakeCfa:
+    """What bench.FiniteAdvance uses of channelflow_amd, for a fake batch."""
+    ERR_NONFINITE = -3
+
+    class CfgError(Exception):
+        def __init__(self, msg, code):
+            super().__init__(msg)
+            self.code = code
+
+    @staticmethod
+    def DNSFlags(**kw):
+        return kw
+
+    @staticmethod
+    def FlowField(*a, **kw):
+        return None
+
+
+class _DivergingBatch:
+    """Members 0..n-1; member `bad` stops being finite once `at` steps have run."""
+
+    def __init__(self, n, bad, at):
+        self.alive, self.bad, self.at, self.steps = set(), bad, at, 0
+
+    def add(self, fields, flags):
+        k = len(self.alive) + 100
+        self.alive.add(k)
+        return k
+
+    def remove(self, k):
+        self.alive.remove(k)
+
+    def finite(self, k):
+        return not (k == self.bad + 100 and self.steps >= self.at)
+
+    def advance(self, n):
+        self.steps += n
+        if any(not self.finite(k) for k in self.alive):
+            raise _FakeCfa.CfgError("velocity not finite after advance", _FakeCfa.ERR_NONFINITE)
+
+
+def test_finite_advance_stops_diverging_member(monkeypatch):
+    """A member found not finite after an advance is removed and recorded with
+    the step count after which it was found; the others keep stepping and are
+    the ones counted."""
+    monkeypatch.setattr(bench, "member_field", lambda m: None)
+    members = [dict(grid=(32, 33, 32), flags={}, norm=0.1 * (i + 1)) for i in range(4)]
+    b = _DivergingBatch(4, bad=2, at=30)
+    adv = bench.FiniteAdvance(_FakeCfa, b, members)
+    for i in range(4):
+        adv.add(i)
+    assert adv(20) == [] and sorted(adv.ids) == [0, 1, 2, 3]
+    assert adv(20) == [2]
+    assert sorted(adv.ids) == [0, 1, 3] and adv.steps == 40
+    assert adv.dropped == [dict(member=2, grid="32x33x32", norm=0.3, nonfinite_after_step=40)]
+    assert adv(20) == [] and adv.points() == 3 * 32 * 33 * 32
+
+
+def test_finite_advance_passes_other_errors(monkeypatch):
+    monkeypatch.setattr(bench, "member_field", lambda m: None)
+
+    class _Failing(_DivergingBatch):
+        def advance(self, n):
+            raise _FakeCfa.CfgError("device error", -2)
+
+    adv = bench.FiniteAdvance(_FakeCfa, _Failing(1, 0, 0), [dict(grid=(16, 17, 16), flags={}, norm=0.1)])
+    adv.add(0)
+    with pytest.raises(_FakeCfa.CfgError):
+        adv(5)
