@@ -28,9 +28,9 @@ The same JSON line also carries
   single_c2   the plain 192x129x192 Poiseuille channel (rotational, SBDF3,
               dt = 0.005) and its per-kernel times;
   graphs      hipGraph captures and replays inside the timed region (one
-              graph per tick for the whole batch), graph launches and host
-              microseconds per tick, and the same steps with one graph per
-              grid class (`per_class`);
+              graph per grid class and tick, the default), graph launches
+              and host microseconds per tick, and the same steps with one
+              graph per tick for the whole batch (`whole_batch`);
   roofline    the dominant kernel of the headline workload (HIP events over
               a profiled pass of the same members right after the timed
               region, one grid class at a time so that a launch's events
@@ -240,7 +240,7 @@ def run_batched(cfa, members, steps, warmup, prof_steps):
     for i in range(len(members)):
         adv.add(i)
     adv(warmup)                           # SMRK2 start + warm-up SBDF3 steps (and graph captures), untimed
-    extra = steady_state(batch, 1, advance=adv)
+    extra = steady_state(batch, batch.size()[1], advance=adv)
     c0, r0 = batch.graph_stats()
     k0, l0, h0 = batch.tick_stats()
     t0 = time.perf_counter()
@@ -252,10 +252,10 @@ def run_batched(cfa, members, steps, warmup, prof_steps):
     timed = sorted(adv.ids)  # the members counted in the timed region
     c1, r1 = batch.graph_stats()
     k1, l1, h1 = batch.tick_stats()
-    # the same steps with one graph per grid class and tick, the classes on
-    # their own streams
-    batch.set_graphs(cfa.Batch.GRAPHS_PER_CLASS)
-    steady_state(batch, batch.size()[1], advance=adv)
+    # the same steps with one graph per tick for the whole batch, the grid
+    # classes as its parallel branches
+    batch.set_graphs(cfa.Batch.GRAPHS_WHOLE)
+    steady_state(batch, 1, advance=adv)
     k2, l2, h2 = batch.tick_stats()
     t2 = time.perf_counter()
     adv(steps)
@@ -284,15 +284,16 @@ def run_batched(cfa, members, steps, warmup, prof_steps):
             raise SystemExit(f"non-finite state of member {i}")
     return dict(value=pts * steps / wall, wall=wall, kt=kt, pts=pts, classes=batch.size()[1],
                 dropped_nonfinite=adv.dropped, steps_run=adv.steps, timed_members=timed,
-                graphs={"mode": "one graph per tick for the whole batch (grid classes as parallel branches)",
+                graphs={"mode": "one graph per grid class and tick, the classes on their own streams (the "
+                                "default)",
                         "captured_in_timed_region": c1 - c0, "replayed_in_timed_region": r1 - r0,
                         "ticks_in_timed_region": k1 - k0, "graph_launches_per_step": (l1 - l0) / steps,
                         "host_us_per_tick": (h1 - h0) / max(1, k1 - k0), "warmup_steps_added": extra,
-                        "per_class": {"mode": "one graph per grid class and tick, the classes on their own "
-                                              "streams; same members and steps, after the timed region",
-                                      "value": pts_w * steps / wall_w, "ms_per_step": wall_w / steps * 1e3,
-                                      "graph_launches_per_step": (l3 - l2) / steps,
-                                      "host_us_per_tick": (h3 - h2) / max(1, k3 - k2)}})
+                        "whole_batch": {"mode": "one graph per tick for the whole batch, the grid classes as "
+                                                "parallel branches; same members and steps, after the timed region",
+                                        "value": pts_w * steps / wall_w, "ms_per_step": wall_w / steps * 1e3,
+                                        "graph_launches_per_step": (l3 - l2) / steps,
+                                        "host_us_per_tick": (h3 - h2) / max(1, k3 - k2)}})
 
 
 def steady_state(batch, per_tick, period=6, max_extra=60, advance=None):

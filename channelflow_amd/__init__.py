@@ -557,9 +557,9 @@ class Batch:
 
     def set_graphs(self, mode=True):
         """hipGraph capture and replay of repeating ticks: False / 0 eager
-        launches, 1 one graph per grid class and tick, True / 2 one graph per
-        tick for the whole batch (the default)."""
-        m = self.GRAPHS_WHOLE if mode is True else (self.GRAPHS_OFF if mode is False else int(mode))
+        launches, True / 1 one graph per grid class and tick (the default),
+        2 one graph per tick for the whole batch."""
+        m = self.GRAPHS_PER_CLASS if mode is True else (self.GRAPHS_OFF if mode is False else int(mode))
         _check(library().cfg_batch_set_graphs(self._h, m))
 
     def graph_stats(self):

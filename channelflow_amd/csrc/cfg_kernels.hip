@@ -528,6 +528,24 @@ constexpr int NXCD = 8;
 // index is renumbered to put a contiguous range of pairs on each XCD, members
 // fastest. The members of a launch share their tau factors, which one XCD
 // then reads from its own L2 for every member after the first.
+// The launch's block (x, y, z) renumbered so that each XCD runs a contiguous
+// range of the linear block order (x fastest): the dispatcher deals
+// consecutive workgroups to the NXCD XCDs in turn, so without it neighbouring
+// blocks, which share the cache lines at their edges, fetch those lines into
+// different L2s. A permutation of the blocks: the work of each is unchanged.
+struct BlockId {
+    int x, y, z;
+};
+__device__ __forceinline__ BlockId xcd_grid() {
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned T = gx * gy * gridDim.z;
+    const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned q = T / NXCD, r = T % NXCD, x = L % NXCD;
+    const unsigned p = x * q + (x < r ? x : r) + L / NXCD;
+    const unsigned t = p / gx;
+    return BlockId{(int)(p - t * gx), (int)(t % gy), (int)(t / gy)};
+}
+
 __device__ __forceinline__ void xcd_block(int nmem, int& mode_block, int& member) {
     const int T = gridDim.x;  // mode blocks x members
     const int L = blockIdx.x;
@@ -1911,6 +1929,7 @@ template <int NX>
 __global__ void __launch_bounds__(256) k_xwinv(Geom g, const double2* __restrict__ twg, int nsrc,
                                                const double2* __restrict__ yp0, long sy, double2* __restrict__ xp0,
                                                long sx) {
+    const BlockId bid = xcd_grid();  // neighbouring blocks on one XCD
     using S = ZW<NX>;
     constexpr int N1 = S::N1, N2 = S::N2;
     extern __shared__ double2 lds[];
@@ -1919,16 +1938,16 @@ __global__ void __launch_bounds__(256) k_xwinv(Geom g, const double2* __restrict
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
     const int nzc = (nkz + S::LPW - 1) / S::LPW;
-    const int wg = blockIdx.x * (blockDim.x / 64) + wave;
+    const int wg = bid.x * (blockDim.x / 64) + wave;
     const int y = wg / nzc;
     if (y >= Ny) return;
     const int L = lane / S::G, j = lane % S::G;
     const int ikz = (wg - y * nzc) * S::LPW + L;
     const bool line_ok = ikz < nkz;
-    const int fo = blockIdx.y;
+    const int fo = bid.y;
     const bool deriv = fo >= nsrc;
     const int fs = deriv ? fo - nsrc : fo;
-    const double2* __restrict__ src = yp0 + blockIdx.z * sy + (((long)fs * Ny + y) * nkz + (line_ok ? ikz : 0)) * nkx;
+    const double2* __restrict__ src = yp0 + bid.z * sy + (((long)fs * Ny + y) * nkz + (line_ok ? ikz : 0)) * nkx;
     const double dk = 2 * PI / g.Lx;
     double2 v[N2];
 #pragma unroll
@@ -1947,7 +1966,7 @@ __global__ void __launch_bounds__(256) k_xwinv(Geom g, const double2* __restrict
     double2 u[N1];
     zw_a2b<NX, +1>(v, u, j, lds + NX + (wave * S::LPW + L) * S::LBUF, lds);
     if (line_ok && j < N2) {
-        double2* __restrict__ out = xp0 + blockIdx.z * sx + ((long)fo * Ny + y) * NX * (long)nkz + ikz;
+        double2* __restrict__ out = xp0 + bid.z * sx + ((long)fo * Ny + y) * NX * (long)nkz + ikz;
 #pragma unroll
         for (int m = 0; m < N1; ++m) out[(long)(j + N2 * m) * nkz] = u[m];
     }
@@ -2315,6 +2334,7 @@ constexpr int yw_waves_fwd(bool divf, int nrt) { return nrt <= 1 ? 4 : (nrt <= (
 template <int METHOD, int NRT>
 __global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((amdgpu_waves_per_eu(yw_waves(NRT))))
     k_yinv_w(Geom g, YMats ym, const NLMember* __restrict__ tab, double2* __restrict__ yp0, long sy) {
+    const BlockId bid = xcd_grid();  // neighbouring blocks on one XCD
     constexpr int NMAT = (METHOD == NL_DIV) ? 1 : 2, NT = 192 * NMAT;
     constexpr int KC = 16, AP = KC + 1, EP = 17;  // chunk columns: 8 even k, then 8 odd k
     constexpr int RS = NRT * 16 + 1;              // LDS rows per matrix: tiles, middle point
@@ -2323,13 +2343,13 @@ __global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((a
     constexpr int SA = NMAT * RS * AP, SE = NMAT * 3 * 2 * 16 * EP;
     constexpr int NSTG = NMAT * RS * KC, NST = (NSTG + NT - 1) / NT;  // staged entries per chunk, per thread
     __shared__ double sm[2 * SA > SE ? 2 * SA : SE];
-    const NLMember& me = tab[blockIdx.y];
-    double2* __restrict__ yp = yp0 + blockIdx.y * sy;
+    const NLMember& me = tab[bid.y];
+    double2* __restrict__ yp = yp0 + bid.y * sy;
     const int Ny = g.Ny, nm = Ny / 2, mid = (Ny & 1) ? nm : -1;
     const long M = g.M;
     const int nrt = (nm + 15) / 16;
     const int w = threadIdx.x >> 6, c = w % 3, am = w / 3, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
-    const long m0 = (long)blockIdx.x * YW_MODES;
+    const long m0 = (long)bid.x * YW_MODES;
     const long mcol = m0 + (lr >> 1);
     const bool mok = mcol < M;
     const gcdouble ucol = gmem(me.u) + 2 * ((long)c * Ny * M + mcol) + (l & 1);
@@ -2471,6 +2491,7 @@ __global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((a
 template <int METHOD, int NRT>
 __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_waves_fwd(METHOD == NL_DIV || METHOD == NL_SKEW, NRT)))) k_yfwd_w(Geom g, YMats ym, int have_g, const double2* __restrict__ fo0,
                                                 long sy, const NLMember* __restrict__ tab) {
+    const BlockId bid = xcd_grid();  // neighbouring blocks on one XCD
     constexpr bool DIVF = (METHOD == NL_DIV || METHOD == NL_SKEW);
     constexpr int NMAT = DIVF ? 2 : 1;
     constexpr int KC = 8, AP = KC + 1, NS = KC / 4;
@@ -2481,13 +2502,13 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
     constexpr int SA = NMAT * RS * AP, NSTG = NMAT * RS * KC, NST = (NSTG + 191) / 192;
     constexpr bool DB = !(DIVF && NRT >= 6);
     __shared__ double sm[(DB ? 2 : 1) * SA];
-    const double* __restrict__ fo = reinterpret_cast<const double*>(fo0 + blockIdx.y * sy);
-    __attribute__((address_space(1))) double* __restrict__ f = gmem_w(tab[blockIdx.y].f);
+    const double* __restrict__ fo = reinterpret_cast<const double*>(fo0 + bid.y * sy);
+    __attribute__((address_space(1))) double* __restrict__ f = gmem_w(tab[bid.y].f);
     const int Ny = g.Ny, nh = (Ny + 1) / 2, nm = Ny / 2;
     const bool odd = Ny & 1;
     const long M = g.M;
     const int c = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4, part = l & 1;
-    const long mcol = (long)blockIdx.x * YW_MODES + (lr >> 1);
+    const long mcol = (long)bid.x * YW_MODES + (lr >> 1);
     const bool mok = mcol < M;
     const double beta = (METHOD == NL_SKEW) ? 0.5 : 1.0;
     double sdx = 0, sdz = 0;  // i k multiplication of the other part, with its sign

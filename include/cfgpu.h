@@ -227,14 +227,15 @@ int cfg_batch_advance_members(cfg_batch* b, int count, const int* members, const
 int cfg_batch_profile(cfg_batch* b, int enable);
 int cfg_batch_kernel_times(cfg_batch* b, double* total_ms, long long* counts);
 /* hipGraph capture of repeating ticks (no counterpart in the reference).
- * mode 0: eager launches; 1: one graph per grid class and tick, each class
- * on its own stream; 2 (the default): one graph per tick for the whole batch,
- * every grid class a parallel branch of it (one launch per tick; the server
- * uses mode 1, since its membership changes every few ticks). A tick whose
- * launch sequence was
- * seen before is captured on its second occurrence and replayed afterwards;
- * results are bit-identical to eager launches in every mode. Profiling
- * switches replay off while it is on. */
+ * mode 0: eager launches; 1 (the default): one graph per grid class and
+ * tick, each class on its own stream, the classes running concurrently;
+ * 2: one graph per tick for the whole batch, every grid class a parallel
+ * branch of it (one launch per tick, but ROCm's launch of such a graph
+ * costs more host time than the per-class launches together, with the same
+ * GPU time). A tick whose launch sequence was seen before is captured on its
+ * second occurrence and replayed afterwards; results are bit-identical to
+ * eager launches in every mode. Profiling switches replay off while it is
+ * on. */
 int cfg_batch_set_graphs(cfg_batch* b, int mode);
 int cfg_batch_graph_stats(const cfg_batch* b, long long* captures, long long* replays);
 /* Ticks run so far (a tick is one nonlinear evaluation of every member below
