@@ -155,6 +155,9 @@ def live(grid):
     return Ny * (2 * (Nx // 3) - 1) * (Nz // 3)  # complex values per component
 
 
+WAVE_FFT_LENGTHS = (16, 32, 48, 64, 96, 128, 192, 256)  # x/z lengths of the wavefront-local FFT kernels
+
+
 def kernel_bytes(grid, method):
     """Compulsory HBM bytes of one member per launch of each kernel (inputs
     read once, outputs written once), SBDF3 main steps."""
@@ -166,12 +169,15 @@ def kernel_bytes(grid, method):
     nyout = {"Rotational": 6, "SkewSymmetric": 6, "Convection": 6, "Divergence": 3}[method]
     nxout = {"Rotational": 6, "SkewSymmetric": 9, "Convection": 9, "Divergence": 3}[method]
     nzout = {"Rotational": 3, "SkewSymmetric": 8, "Convection": 3, "Divergence": 6}[method]
+    # the skew-symmetric forward x-FFT of the wavefront lengths adds the
+    # x-derivative part of the divergence and writes f_c and u_c u_1: 6 fields
+    nfo = 6 if method == "SkewSymmetric" and Nx in WAVE_FFT_LENGTHS else nzout
     return {
         "k_yinv": (3 + nyout) * L,
         "k_xinv": nyout * L + nxout * X,
         "k_zpass": nxout * X + nzout * X,
-        "k_xfwd": nzout * X + nzout * L,
-        "k_yfwd": nzout * L + 3 * L,
+        "k_xfwd": nzout * X + nfo * L,
+        "k_yfwd": nfo * L + 3 * L,
         # SBDF3 right-hand side u_n..u_{n-2}, f_n..f_{n-2} read, u (3) and q (1)
         # written: 22 L. The per-mode tau factors are shared by every member of a
         # class and are overhead (tau_factor_bytes), not algorithmic bytes.

@@ -1972,11 +1972,18 @@ __global__ void __launch_bounds__(256) k_xwinv(Geom g, const double2* __restrict
     }
 }
 
-// forward x-FFT: zo [G][Ny][Nx][nkz] -> fo [G][Ny][nkz][nkx], scaled by 1/(Nx Nz)
+// forward x-FFT: zo [G][Ny][Nx][nkz] -> fo [G][Ny][nkz][nkx], scaled by 1/(Nx Nz).
+// fold (the skew-symmetric form): the x-derivative part of its divergence is
+// added here, in x-spectral space, and the launch has 5 line tasks, not 8:
+// task c < 3 writes field c as f_c + 1/2 i kx (u_c u_0) (u_c u_0 is z-pass
+// field 3 + c), task 1 also u_0u_1 as field 3, tasks 3 and 4 u_1u_1 (z-pass
+// field 6) and u_1u_2 (7) as fields 4 and 5. The y-forward transform then
+// reads f_c and u_c u_1 only. The added term is formed exactly as the
+// y-forward transform formed it before (the same products in the same order).
 template <int NX>
 __global__ void __launch_bounds__(256) k_xwfwd(Geom g, const double2* __restrict__ twg,
                                                const double2* __restrict__ zo0, long sx, double2* __restrict__ fo0,
-                                               long sy) {
+                                               long sy, int fold) {
     using S = ZW<NX>;
     constexpr int N1 = S::N1, N2 = S::N2;
     extern __shared__ double2 lds[];
@@ -1991,24 +1998,48 @@ __global__ void __launch_bounds__(256) k_xwfwd(Geom g, const double2* __restrict
     const int L = lane / S::G, j = lane % S::G;
     const int ikz = (wg - y * nzc) * S::LPW + L;
     const bool line_ok = ikz < nkz;
-    const int f = blockIdx.y;
-    const double2* __restrict__ in =
-        zo0 + blockIdx.z * sx + ((long)f * Ny + y) * NX * (long)nkz + (line_ok ? ikz : 0);
-    double2 u[N1];
+    const int t = blockIdx.y;
+    int fa = t, oa = t, fb = -1, ob = -1;
+    if (fold) {
+        fa = t < 3 ? t : t + 3;
+        oa = t < 3 ? t : t + 1;
+        if (t < 3) fb = 3 + t;
+        if (t == 1) ob = 3;
+    }
+    double2* const T = lds + NX + (wave * S::LPW + L) * S::LBUF;
+    auto line = [&](int f, double2 (&v)[N2]) {
+        const double2* __restrict__ in =
+            zo0 + blockIdx.z * sx + ((long)f * Ny + y) * NX * (long)nkz + (line_ok ? ikz : 0);
+        double2 u[N1];
 #pragma unroll
-    for (int m = 0; m < N1; ++m) u[m] = (line_ok && j < N2) ? in[(long)(j + N2 * m) * nkz] : make_double2(0.0, 0.0);
-    double2 v[N2];
-    zw_b2a<NX, -1>(u, v, j, lds + NX + (wave * S::LPW + L) * S::LBUF, lds);
+        for (int m = 0; m < N1; ++m)
+            u[m] = (line_ok && j < N2) ? in[(long)(j + N2 * m) * nkz] : make_double2(0.0, 0.0);
+        zw_b2a<NX, -1>(u, v, j, T, lds);
+    };
+    double2 va[N2], vb[N2];
+    line(fa, va);
+    if (fb >= 0) line(fb, vb);
     if (line_ok && j < N1) {
-        const double scale = 1.0 / ((double)g.Nx * g.Nz);
-        double2* __restrict__ out = fo0 + blockIdx.z * sy + (((long)f * Ny + y) * nkz + ikz) * nkx;
+        const double scale = 1.0 / ((double)g.Nx * g.Nz), beta = 0.5;
+        double2* __restrict__ base = fo0 + blockIdx.z * sy;
+        double2* __restrict__ outa = base + (((long)oa * Ny + y) * nkz + ikz) * nkx;
+        double2* __restrict__ outb = base + (((long)(ob < 0 ? oa : ob) * Ny + y) * nkz + ikz) * nkx;
 #pragma unroll
         for (int p = 0; p < N2; ++p) {
             const int n = j + N1 * p;
             int ikx = -1;
             if (n <= g.kxd) ikx = n;
             else if (n >= NX - g.kxd) ikx = n - NX + nkx;
-            if (ikx >= 0) out[ikx] = make_double2(v[p].x * scale, v[p].y * scale);
+            if (ikx < 0) continue;
+            double2 a = make_double2(va[p].x * scale, va[p].y * scale);
+            if (fb >= 0) {
+                const double2 b = make_double2(vb[p].x * scale, vb[p].y * scale);
+                const int kx = kx_of(g, ikx);
+                const double s0 = -beta * 2 * PI * kx / g.Lx, s1 = beta * 2 * PI * kx / g.Lx;
+                a = make_double2(fma(s0, b.y, a.x), fma(s1, b.x, a.y));  // a + beta i kx b
+                if (ob >= 0) outb[ikx] = b;
+            }
+            outa[ikx] = a;
         }
     }
 }
@@ -2132,7 +2163,7 @@ __global__ void __launch_bounds__(256) k_yinv_t(Geom g, YMats ym, const NLMember
 // y values -> Chebyshev coefficients (same outputs as k_yfwd): f_i = Ti G_i + DTi H_i.
 template <int METHOD>
 __global__ void __launch_bounds__(256) k_yfwd_t(Geom g, YMats ym, int have_g, const double2* __restrict__ fo0,
-                                                long sy, const NLMember* __restrict__ tab) {
+                                                long sy, const NLMember* __restrict__ tab, int xfold) {
     constexpr bool DIVF = (METHOD == NL_DIV || METHOD == NL_SKEW);
     constexpr int NX = DIVF ? 6 : 3;
     __shared__ double2 Xs[NX][YB_K][YB_M];
@@ -2161,7 +2192,10 @@ __global__ void __launch_bounds__(256) k_yfwd_t(Geom g, YMats ym, int have_g, co
             double2 G = make_double2(0.0, 0.0), H = make_double2(0.0, 0.0);
             if (y < Ny && m < M) {
                 if (have_g) G = fo[((long)c * Ny + y) * M + m];
-                if (DIVF) {
+                if (DIVF && METHOD == NL_SKEW && xfold) {  // x and z terms already in g_c; uu_c1 is field 3 + c
+                    const double2 x1 = fo[((long)(3 + c) * Ny + y) * M + m];
+                    H = make_double2(beta * x1.x, beta * x1.y);
+                } else if (DIVF) {
                     const int ikx = m % g.nkx, ikz = m / g.nkx;
                     const double dx = 2 * PI * kx_of(g, ikx) / g.Lx, dz = 2 * PI * ikz / g.Lz;
                     const int uuf[3][3] = {{3, 4, 5}, {4, 6, 7}, {5, 7, 8}};
@@ -2490,7 +2524,7 @@ __global__ void __launch_bounds__(METHOD == NL_DIV ? 192 : 384) __attribute__((a
 // coefficients of each parity.
 template <int METHOD, int NRT>
 __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_waves_fwd(METHOD == NL_DIV || METHOD == NL_SKEW, NRT)))) k_yfwd_w(Geom g, YMats ym, int have_g, const double2* __restrict__ fo0,
-                                                long sy, const NLMember* __restrict__ tab) {
+                                                long sy, const NLMember* __restrict__ tab, int xfold) {
     const BlockId bid = xcd_grid();  // neighbouring blocks on one XCD
     constexpr bool DIVF = (METHOD == NL_DIV || METHOD == NL_SKEW);
     constexpr int NMAT = DIVF ? 2 : 1;
@@ -2519,9 +2553,13 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
         sdz = sg * 2 * PI * ikz / g.Lz;
     }
     // field of uu_cj in fo: 00:3 01:4 02:5 11:6 12:7 22:8. The skew-symmetric
-    // z pass has already added beta ikz uu_c2 to g_c (and keeps no uu22).
+    // z pass has already added beta ikz uu_c2 to g_c (and keeps no uu22); with
+    // xfold the forward x-FFT has added beta ikx uu_c0 too, and uu_c1 is field
+    // 3 + c.
     constexpr bool ZTERM = (METHOD == NL_DIV);
-    const int f0 = c == 0 ? 3 : (c == 1 ? 4 : 5), f1 = c == 0 ? 4 : (c == 1 ? 6 : 7), f2 = c == 0 ? 5 : (c == 1 ? 7 : 8);
+    const bool XTERM = !(METHOD == NL_SKEW && xfold);
+    const int f0 = c == 0 ? 3 : (c == 1 ? 4 : 5), f2 = c == 0 ? 5 : (c == 1 ? 7 : 8);
+    const int f1 = XTERM ? (c == 0 ? 4 : (c == 1 ? 6 : 7)) : 3 + c;
     const long rowst = 2 * M;
     const double* __restrict__ pg = fo + 2 * ((long)c * Ny * M + mcol) + part;
     const double* __restrict__ p0 = fo + 2 * ((long)f0 * Ny * M + mcol) + (1 - part);
@@ -2533,7 +2571,7 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
     double dlast = 0.0;
     auto gval = [&](int y) {
         double G = have_g ? pg[y * rowst] : 0.0;
-        if (DIVF) G += ZTERM ? sdx * p0[y * rowst] + sdz * p2[y * rowst] : sdx * p0[y * rowst];
+        if (DIVF && XTERM) G += ZTERM ? sdx * p0[y * rowst] + sdz * p2[y * rowst] : sdx * p0[y * rowst];
         return G;
     };
     double sgn[NS], dgn[NS], shn[NS], dhn[NS];
@@ -2662,20 +2700,22 @@ static void launch_yinv(const Geom& g, const YMats& ym, int n, const NLMember* t
     }
 }
 
+// xfold: the skew-symmetric fields come folded from k_xwfwd (f_c with its
+// x-derivative term in fields 0-2, u_cu_1 in 3 + c)
 template <int METHOD>
 static void launch_yfwd(const Geom& g, const YMats& ym, int n, int have_g, const double2* fo, long sy,
-                        const NLMember* tab, hipStream_t st) {
-    const int nrt = (g.Ny / 2 + 15) / 16;
+                        const NLMember* tab, hipStream_t st, bool xfold = false) {
+    const int nrt = (g.Ny / 2 + 15) / 16, xf = xfold ? 1 : 0;
     const dim3 grid((unsigned)((g.M + YW_MODES - 1) / YW_MODES), n);
-    if (nrt <= 1) k_yfwd_w<METHOD, 1><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
-    else if (nrt <= 2) k_yfwd_w<METHOD, 2><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
-    else if (nrt <= 3) k_yfwd_w<METHOD, 3><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
-    else if (nrt <= 4) k_yfwd_w<METHOD, 4><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
-    else if (nrt <= 6) k_yfwd_w<METHOD, 6><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
-    else if (nrt <= 8) k_yfwd_w<METHOD, 8><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab);
+    if (nrt <= 1) k_yfwd_w<METHOD, 1><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab, xf);
+    else if (nrt <= 2) k_yfwd_w<METHOD, 2><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab, xf);
+    else if (nrt <= 3) k_yfwd_w<METHOD, 3><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab, xf);
+    else if (nrt <= 4) k_yfwd_w<METHOD, 4><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab, xf);
+    else if (nrt <= 6) k_yfwd_w<METHOD, 6><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab, xf);
+    else if (nrt <= 8) k_yfwd_w<METHOD, 8><<<grid, 192, 0, st>>>(g, ym, have_g, fo, sy, tab, xf);
     else {
         const dim3 tg((unsigned)(((g.M + YB_M - 1) / YB_M) * ((g.Ny + YB_R - 1) / YB_R)), n);
-        k_yfwd_t<METHOD><<<tg, 256, 0, st>>>(g, ym, have_g, fo, sy, tab);
+        k_yfwd_t<METHOD><<<tg, 256, 0, st>>>(g, ym, have_g, fo, sy, tab, xf);
     }
 }
 
@@ -2715,23 +2755,28 @@ static int xfft_lines(const Geom& g, const NLTune& tn) {
 
 #define CFG_FFT_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 
+// fold (forward only): the skew-symmetric layout of k_xwfwd (8 z-pass fields in,
+// 6 out, the x-derivative part of the divergence added); only the wavefront
+// FFT lengths have it. Returns whether it folded.
 template <int NX>
-static void launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, int n, const double2* in,
-                     long sin, double2* out, long sout, const NLTune& tn, hipStream_t st) {
+static bool launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, int n, const double2* in,
+                     long sin, double2* out, long sout, const NLTune& tn, hipStream_t st, bool fold) {
     if constexpr (zwave_size(NX)) {
         using S = ZW<NX>;
         const int wb = tn.xwb > 0 ? tn.xwb : 4;
         const long waves = (long)g.Ny * ((g.nkz + S::LPW - 1) / S::LPW);
         const size_t lds = (size_t)(NX + wb * S::LPW * S::LBUF) * sizeof(double2);
-        const dim3 grid((unsigned)((waves + wb - 1) / wb), nf, n);
         if (inverse) {
+            const dim3 grid((unsigned)((waves + wb - 1) / wb), nf, n);
             allow_lds(k_xwinv<NX>, lds);
             k_xwinv<NX><<<grid, 64 * wb, lds, st>>>(g, px.tw, nsrc, in, sin, out, sout);
         } else {
+            if (fold && nf != 8) throw std::runtime_error("x-FFT fold: the skew-symmetric layout has 8 fields");
+            const dim3 grid((unsigned)((waves + wb - 1) / wb), fold ? 5 : nf, n);
             allow_lds(k_xwfwd<NX>, lds);
-            k_xwfwd<NX><<<grid, 64 * wb, lds, st>>>(g, px.tw, in, sin, out, sout);
+            k_xwfwd<NX><<<grid, 64 * wb, lds, st>>>(g, px.tw, in, sin, out, sout, fold ? 1 : 0);
         }
-        return;
+        return fold && !inverse;
     }
     const int XL = xfft_lines(g, tn);
     const int nchunk = (g.nkz + XL - 1) / XL;
@@ -2744,17 +2789,19 @@ static void launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, i
         allow_lds(k_xfwd<NX>, lds);
         k_xfwd<NX><<<grid, 256, lds, st>>>(g, px, XL, in, sin, out, sout);
     }
+    return false;
 }
 
-// in/out strides are per member, in double2
-static void launch_xfft(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, int n, const double2* in,
-                        long sin, double2* out, long sout, const NLTune& tn, hipStream_t st) {
+// in/out strides are per member, in double2. Returns whether the forward
+// transform folded the skew-symmetric layout (launch_x).
+static bool launch_xfft(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, int nf, int n, const double2* in,
+                        long sin, double2* out, long sout, const NLTune& tn, hipStream_t st, bool fold = false) {
     switch (g.Nx) {
 #define CASE_X(N) \
-    case N: launch_x<N>(inverse, g, px, nsrc, nf, n, in, sin, out, sout, tn, st); return;
+    case N: return launch_x<N>(inverse, g, px, nsrc, nf, n, in, sin, out, sout, tn, st, fold);
         CFG_FFT_SIZES(CASE_X)
 #undef CASE_X
-        default: launch_x<0>(inverse, g, px, nsrc, nf, n, in, sin, out, sout, tn, st);
+        default: return launch_x<0>(inverse, g, px, nsrc, nf, n, in, sin, out, sout, tn, st, fold);
     }
 }
 
@@ -2884,7 +2931,10 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     // skew-symmetric one; one launch when both exist (the fields are
     // contiguous)
     const bool uu = method == NL_DIV || method == NL_SKEW;
-    if (want_g) launch_xfft(false, g, px, 0, method == NL_SKEW ? 8 : (uu ? 9 : 3), n, zo, sx, fo, sy, tn, st);
+    bool xfold = false;  // the skew-symmetric x-derivative terms already in fields 0-2 (k_xwfwd)
+    if (want_g)
+        xfold = launch_xfft(false, g, px, 0, method == NL_SKEW ? 8 : (uu ? 9 : 3), n, zo, sx, fo, sy, tn, st,
+                            method == NL_SKEW);
     else if (uu) {
         const long off = 3L * g.Ny * g.Nx * g.nkz;
         const long offo = 3L * g.Ny * g.M;
@@ -2895,7 +2945,7 @@ void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     switch (method) {
         case NL_ROT: launch_yfwd<NL_ROT>(g, ym, n, 1, fo, sy, tab, st); break;
         case NL_CONV: launch_yfwd<NL_CONV>(g, ym, n, 1, fo, sy, tab, st); break;
-        case NL_SKEW: launch_yfwd<NL_SKEW>(g, ym, n, 1, fo, sy, tab, st); break;
+        case NL_SKEW: launch_yfwd<NL_SKEW>(g, ym, n, 1, fo, sy, tab, st, xfold); break;
         default: launch_yfwd<NL_DIV>(g, ym, n, want_g, fo, sy, tab, st);
     }
     T.end(K_YFWD, ev);
