@@ -2763,7 +2763,7 @@ static bool launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, i
                      long sin, double2* out, long sout, const NLTune& tn, hipStream_t st, bool fold) {
     if constexpr (zwave_size(NX)) {
         using S = ZW<NX>;
-        const int wb = tn.xwb > 0 ? tn.xwb : 4;
+        const int wb = (!inverse && tn.xwbf > 0) ? tn.xwbf : (tn.xwb > 0 ? tn.xwb : 4);
         const long waves = (long)g.Ny * ((g.nkz + S::LPW - 1) / S::LPW);
         const size_t lds = (size_t)(NX + wb * S::LPW * S::LBUF) * sizeof(double2);
         if (inverse) {
@@ -2996,8 +2996,8 @@ NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
         if (std::getenv("CFGPU_TUNE_LOG")) {
             std::fprintf(stderr, "cfg tune %s %dx%dx%d form %d members %d:", what, g.Nx, g.Ny, g.Nz, method, n);
             for (size_t i = 0; i < cands.size(); ++i)
-                std::fprintf(stderr, " [zwb %d zstage %d zxb %d xwb %d xl %d] %.1f us%s", cands[i].zwb,
-                             cands[i].zstage, cands[i].zxb, cands[i].xwb, cands[i].xl, best[i] * 1e3f,
+                std::fprintf(stderr, " [zwb %d zstage %d zxb %d xwb %d xwbf %d xl %d] %.1f us%s", cands[i].zwb,
+                             cands[i].zstage, cands[i].zxb, cands[i].xwb, cands[i].xwbf, cands[i].xl, best[i] * 1e3f,
                              i == k ? " *" : "");
             std::fprintf(stderr, "\n");
         }
@@ -3054,6 +3054,19 @@ NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
         }, "x");
         out.xwb = x.xwb;
         out.xl = x.xl;
+    }
+    if (zwave_size(g.Nx) && want_g) {  // the forward wavefront x-FFT on its own (it folds the skew form)
+        const int nzf = method == NL_SKEW ? 8 : (method == NL_DIV ? 9 : 3);  // as launch_nonlinear
+        std::vector<NLTune> fc;
+        for (int wb = 1; wb <= 4; wb *= 2) {
+            NLTune c;
+            c.xwbf = wb;
+            fc.push_back(c);
+        }
+        const NLTune f = pick(fc, [&](const NLTune& c) {
+            launch_xfft(false, g, px, 0, nzf, n, zo, sx, yp, sy, c, st, method == NL_SKEW);
+        }, "xfwd");
+        out.xwbf = f.xwbf;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

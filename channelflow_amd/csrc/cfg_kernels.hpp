@@ -214,6 +214,7 @@ struct NLTune {
     int zwb = 0;      // wavefronts per block of the unstaged wavefront z-pass (0: 4)
     int zstage = -1;  // wavefront z-pass stages its inputs in LDS: 1 / 0, or -1 the grid's rule
     int xwb = 0;  // wavefronts per block of the wavefront x-FFTs (0: 4)
+    int xwbf = 0; // the same for the forward wavefront x-FFT (0: xwb)
 };
 
 void launch_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const FFTPlan& pz, int method, int n,
