@@ -1,10 +1,11 @@
 """cfg_nonlinear_noalloc (include/cfgpu.h), the entry point that allocates no
 device memory, against cfg_nonlinear, the allocating entry point it varies:
 the same NSE::nonlinear evaluation (nse.cpp:383-391), so the live modes must
-agree bit for bit. Two calls in a row on one device, the second on a smaller
+agree bit for bit. Calls in a row on one device, the second on a smaller
 grid than the first (the shared scratch array then holds the first call's
-data), and a grid beyond the scratch array's size rejected with a status
-code before any device work."""
+data), the last on 64x65x64, the largest C3 grid, whose z pass stages its
+inputs in LDS; and a grid beyond the scratch array's size rejected with a
+status code before any device work."""
 import ctypes as C
 
 import numpy as np
@@ -70,7 +71,8 @@ def call_noalloc(grid, flags, uc):
 
 @pytest.mark.parametrize("calls", [[((48, 49, 48), "SkewSymmetric", "DealiasXZ"),
                                     ((16, 17, 16), "Rotational", "DealiasXZ"),
-                                    ((24, 25, 18), "Convection", "NoDealiasing")]])
+                                    ((24, 25, 18), "Convection", "NoDealiasing"),
+                                    ((64, 65, 64), "SkewSymmetric", "DealiasXZ")]])
 def test_noalloc_matches_nonlinear(gpu, calls):
     for k, (shape, method, dealias) in enumerate(calls):
         Nx, Ny, Nz = shape
