@@ -1715,15 +1715,23 @@ __device__ __forceinline__ void zw_r2c(const ZWCtx& c, double2* __restrict__ zo,
 // wavefronts per block of the staged wavefront z pass
 constexpr int ZWAVE_WB = 2;
 
-// The z pass stages its inputs at Nz = 64 only. The choice depends on the grid
-// alone (never on the member count), so a member's result is the same alone
-// and in a batch: the two forms contract multiply-adds differently (the
-// compiler fuses either product of a*b + c*d depending on the code around
-// it) and differ in the last bits. At Nz = 64 the staged form wins where it
-// matters, in batches (16 members of 64x65x64, skew-symmetric: 104 against
-// 122 us per launch) and loses 2.6 us on a lone field; at 32 and 48 (24
-// members) and for lone 128^3 and 192^3 fields the unstaged one is faster.
-constexpr bool zwave_staged(int nz) { return nz == 64; }
+// The z pass stages its inputs at Nz = 64, and at Nz = 128 for the rotational
+// form. The choice depends on the grid and form alone (never on the member
+// count), so a member's result is the same alone and in a batch: the two forms
+// contract multiply-adds differently (the compiler fuses either product of
+// a*b + c*d depending on the code around it) and differ in the last bits. At
+// Nz = 64 the staged form wins where it matters, in batches (16 members of
+// 64x65x64, skew-symmetric: 104 against 122 us per launch) and loses 2.6 us on
+// a lone field; at 32 and 48 (24 members) and for lone 96^3 and 192^3 fields
+// the unstaged one is faster. The rotational form at Nz = 128 runs the staged
+// kernel at 2 waves per SIMD (zwave_min_waves; 256 VGPRs and a 60-byte spill,
+// where the unstaged one needs 277 + 21 and runs 1): a lone 128^3 field 56.4
+// -> 47.5 us. The same cap on the skew-symmetric form spills 232-692 bytes at
+// 96-128 and on the rotational one 200 bytes at 192, and those were slower.
+constexpr bool zwave_staged(int method, int nz) { return nz == 64 || (nz == 128 && method == NL_ROT); }
+constexpr int zwave_min_waves(int method, int nz, bool staged) {
+    return (staged && nz == 128 && method == NL_ROT) ? 2 : 1;
+}
 
 // The z pass of k_zpass with one wavefront per 64/G lines (same fields, same
 // products in the same order). A block is independent wavefronts: any number
@@ -1731,9 +1739,10 @@ constexpr bool zwave_staged(int nz) { return nz == 64; }
 // the next transform's two input fields in LDS while it works on the current
 // one. Staging hides the loads' latency where many wavefronts share a CU;
 // with few wavefronts per SIMD the extra LDS round trip costs more than it
-// hides (zwave_staged picks the form per grid).
+// hides (zwave_staged picks the form per grid and nonlinear form).
 template <int METHOD, int NZ, bool STAGED>
-__global__ void __launch_bounds__(256) k_zwave(Geom g, const double2* __restrict__ twg, const NLMember* __restrict__ tab,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(zwave_min_waves(METHOD, NZ, STAGED), 8)))
+k_zwave(Geom g, const double2* __restrict__ twg, const NLMember* __restrict__ tab,
                                                int want_g, const double2* __restrict__ xp0, double2* __restrict__ zo0,
                                                long sx) {
     using S = ZW<NZ>;
@@ -2811,7 +2820,7 @@ static void launch_z_t(const Geom& g, const FFTPlan& pz, int n, const NLMember* 
     if constexpr (zwave_size(NZ)) {
         using S = ZW<NZ>;
         const long waves = (long)g.Ny * ((g.Nx + S::LPW - 1) / S::LPW);
-        if (tn.zstage < 0 ? zwave_staged(NZ) : tn.zstage != 0) {
+        if (tn.zstage < 0 ? zwave_staged(METHOD, NZ) : tn.zstage != 0) {
             if (g.nkz > NZ / 2) throw std::runtime_error("z pass: more live kz bins than Nz/2");
             const dim3 grid((unsigned)((waves + ZWAVE_WB - 1) / ZWAVE_WB), n);
             k_zwave<METHOD, NZ, true><<<grid, 64 * ZWAVE_WB, 0, st>>>(g, pz.tw, tab, want_g, xp, zo, sx);
@@ -3006,9 +3015,9 @@ NLTune tune_nonlinear(const Geom& g, const YMats& ym, const FFTPlan& px, const F
     NLTune out;
     std::vector<NLTune> zc;
     if (zwave_size(g.Nz)) {  // wavefront z pass: the staged form, or wavefronts per block
-        // the grid's form (zwave_staged), or the one CFGPU_ZWAVE_STAGED=1 / 0 names
+        // the grid's and form's choice (zwave_staged), or the one CFGPU_ZWAVE_STAGED=1 / 0 names
         const char* force = std::getenv("CFGPU_ZWAVE_STAGED");
-        const bool staged = force ? force[0] != '0' : zwave_staged(g.Nz), unstaged = !staged;
+        const bool staged = force ? force[0] != '0' : zwave_staged(method, g.Nz), unstaged = !staged;
         NLTune c;
         if (staged) {
             c.zwb = ZWAVE_WB;

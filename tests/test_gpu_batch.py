@@ -214,12 +214,14 @@ def test_zpass_staged_and_unstaged_agree(gpu, monkeypatch):
     velocity's size, and 1e-12 of the pressure's, which the tau correction
     makes sensitive to rounding (a misplaced or stale staged bin would be off
     by the field's own size, 1e-3 and more). Members cover the four
-    nonlinear forms, the wavefront lengths 16, 32, 48 and 64, and grids without
-    dealiasing (Nz/2 live kz bins, the staging's largest)."""
+    nonlinear forms, the wavefront lengths 16, 32, 48 and 64, the rotational
+    form at 128 (whose staged kernel is compiled for 2 waves per SIMD), and
+    grids without dealiasing (Nz/2 live kz bins, the staging's largest)."""
     specs = [((16, 17, 16), "SkewSymmetric", "DealiasXZ"), ((32, 17, 32), "Rotational", "DealiasXZ"),
              ((48, 17, 48), "Convection", "DealiasXZ"), ((64, 17, 64), "SkewSymmetric", "DealiasXZ"),
              ((32, 17, 32), "Divergence", "NoDealiasing"), ((48, 17, 48), "SkewSymmetric", "NoDealiasing"),
-             ((64, 17, 64), "Rotational", "DealiasXZ"), ((64, 17, 64), "Convection", "NoDealiasing")]
+             ((64, 17, 64), "Rotational", "DealiasXZ"), ((64, 17, 64), "Convection", "NoDealiasing"),
+             ((128, 17, 128), "Rotational", "DealiasXZ")]
 
     def run():
         b = cfa.Batch()
