@@ -1713,7 +1713,7 @@ __device__ __forceinline__ void zw_r2c(const ZWCtx& c, double2* __restrict__ zo,
 }
 
 // wavefronts per block of the staged wavefront z pass
-constexpr int ZWAVE_WB = 2;
+constexpr int ZWAVE_WB = 1;
 
 // The z pass stages its inputs at Nz = 64, and at Nz = 128 for the rotational
 // form. The choice depends on the grid and form alone (never on the member
