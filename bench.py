@@ -607,8 +607,8 @@ def main():
                               cmd)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(ensembles shard by member across GPUs; see the sharding notes of the design)", file=sys.stderr)
+        print(f"bench.py: the benchmark runs on one GPU; --gpus {args.gpus} is not supported "
+              "(ensembles would shard by member across GPUs, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     members = c3_members() if args.workload == "c3" else c4_members()

@@ -2,7 +2,7 @@
 // the parity oracle and the timed CPU baseline.
 //
 // TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's
-// cpu_baseline leg may load this library, and only as the checker. The
+// cpu_baseline leg may load this library, and only to check results against. The
 // product (channelflow_amd, libcfgpu.so) never links or calls it.
 //
 // Every function cites the reference file:line it restates (paths relative to

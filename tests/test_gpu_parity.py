@@ -1,6 +1,6 @@
 """GPU parity: the HIP path through the C ABI against the oracle (oracle/cpu_ref.cpp).
 
-Tolerances (north star): relative L2 <= 1e-12 per velocity component for one
+Tolerances, the bar the FP64 path is held to: relative L2 <= 1e-12 per velocity component for one
 nonlinear-term or one tau-solve call; <= 1e-10 after 20 time steps; the
 reference's own golden trajectory (tests/timeIntegrationTest.cpp) at its own
 bar, L2Dist <= 1e-13.
