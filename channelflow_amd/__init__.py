@@ -27,7 +27,8 @@ _ENUMS = dict(baseflow=BASEFLOW, constraint=CONSTRAINT, timestepping=STEPPING, i
 EXPORTED = ("cfg_last_error", "cfg_device_count", "cfg_dns_create", "cfg_dns_destroy", "cfg_dns_advance",
             "cfg_dns_set_state", "cfg_dns_get_state", "cfg_dns_reset_dt", "cfg_dns_reset_time",
             "cfg_dns_reset_gradp", "cfg_dns_reset_bulkv", "cfg_dns_time", "cfg_dns_dpdx", "cfg_dns_ubulk",
-            "cfg_dns_cfl", "cfg_dns_base", "cfg_dns_device_state", "cfg_dns_profile", "cfg_dns_kernel_times",
+            "cfg_dns_cfl", "cfg_dns_base", "cfg_dns_device_state", "cfg_dns_profile", "cfg_dns_set_graphs",
+            "cfg_dns_kernel_times",
             "cfg_kernel_name", "cfg_nonlinear", "cfg_nonlinear_noalloc", "cfg_solve", "cfg_pressure",
             "cfg_batch_create", "cfg_batch_destroy", "cfg_batch_add", "cfg_batch_remove", "cfg_batch_advance",
             "cfg_batch_size", "cfg_batch_get_state", "cfg_batch_set_state", "cfg_batch_time", "cfg_batch_finite",
@@ -95,6 +96,7 @@ def library():
         "cfg_dns_base": ([v, P, P], i),
         "cfg_dns_device_state": ([v, C.POINTER(P), C.POINTER(P), C.POINTER(C.c_int), C.POINTER(C.c_int)], i),
         "cfg_dns_profile": ([v, i], i),
+        "cfg_dns_set_graphs": ([v, i], i),
         "cfg_dns_kernel_times": ([v, P, C.POINTER(C.c_longlong)], i),
         "cfg_kernel_name": ([i], C.c_char_p),
         "cfg_nonlinear": ([G, F, P, P], i),
@@ -377,6 +379,11 @@ class DNS:
     def profile(self, enable=True):
         """Enable (and clear) per-kernel HIP-event timing on this DNS's stream."""
         _check(library().cfg_dns_profile(self._h, int(bool(enable))))
+
+    def set_graphs(self, enable=True):
+        """hipGraph replay of repeating steps (off by default: for one field it
+        is no faster than eager launches; bit-identical to them)."""
+        _check(library().cfg_dns_set_graphs(self._h, int(bool(enable))))
 
     def kernel_times(self):
         """{kernel name: (total ms, launches)} since profile(True)."""

@@ -159,6 +159,13 @@ int cfg_dns_device_state(cfg_dns* dns, double** u_dev, double** q_dev, int* nkx,
     });
 }
 
+int cfg_dns_set_graphs(cfg_dns* dns, int enable) {
+    return guarded([&] {
+        if (!dns) throw Error(CFG_ERR_ARG, "null dns");
+        dns->ex->set_graphs(enable != 0);
+    });
+}
+
 int cfg_dns_profile(cfg_dns* dns, int enable) {
     return guarded([&] {
         if (!dns) throw Error(CFG_ERR_ARG, "null dns");

@@ -189,6 +189,8 @@ class DNS {
         check(cfg_dns_reset_dt(h_, dt), "cfg_dns_reset_dt");
         flags_.dt = dt;
     }
+    /* hipGraph replay of repeating steps: off by default, bit-identical to eager */
+    void set_graphs(bool on) { check(cfg_dns_set_graphs(h_, on ? 1 : 0), "cfg_dns_set_graphs"); }
     void reset_time(Real t) { check(cfg_dns_reset_time(h_, t), "cfg_dns_reset_time"); }
     void reset_gradp(Real dPdx, Real dPdz) { check(cfg_dns_reset_gradp(h_, dPdx, dPdz), "cfg_dns_reset_gradp"); }
     void reset_bulkv(Real Ubulk, Real Wbulk) { check(cfg_dns_reset_bulkv(h_, Ubulk, Wbulk), "cfg_dns_reset_bulkv"); }
