@@ -36,13 +36,13 @@ struct BatchMember {
     bool finite = true;  // state finite after its last advance
 };
 
-// Graph modes of a batch: none (eager launches), one graph per grid class and
-// tick (each class on its own stream), or one graph per tick for the whole
-// batch (the default), its classes captured as parallel branches forked from
-// the batch's stream: one launch per tick, and the branches overlap on the
-// device as the class streams do. A server's membership changes every few
-// ticks, and a whole-batch key with it, so the server keeps one graph per
-// class.
+// hipGraph replay of repeating ticks: none (eager launches), one graph per
+// grid class and tick, each class on its own stream (the default of a Batch),
+// or one graph per tick for the whole batch, its classes captured as parallel
+// branches forked from the batch's stream (one launch per tick, but more host
+// time per tick on ROCm, with the same device time). A server's membership
+// changes every few ticks, and its keys with it, so the server launches
+// eagerly.
 enum GraphMode { GRAPHS_OFF = 0, GRAPHS_PER_CLASS = 1, GRAPHS_WHOLE = 2 };
 
 class Batch {
@@ -462,7 +462,12 @@ class AdmissionPlan {
 // Results come back through pinned host memory without stalling the others.
 class Server {
    public:
-    explicit Server(double budget) : budget_(budget) { batch_.set_graphs(GRAPHS_PER_CLASS); }
+    // Eager launches: the membership changes every few ticks as requests come
+    // and go, so a captured tick is rarely replayed often enough to pay for its
+    // capture (96 requests, Poisson 8/s: p50 0.42 s eager against 0.51 s with
+    // per-class graphs, 4.97e9 against 4.95e9 grid-pt*steps/s; all queued at
+    // t = 0: 6.43e9 against 6.38e9).
+    explicit Server(double budget) : budget_(budget) { batch_.set_graphs(GRAPHS_OFF); }
     ~Server() {
         batch_.sync();
         for (auto& kv : done_)
