@@ -149,7 +149,8 @@ def test_dns_c1_20_steps(gpu):
 
 @pytest.mark.parametrize("method,stepping,init", [("Rotational", "SBDF2", "SMRK2"), ("Convection", "SBDF4", "SMRK2"),
                                                   ("Divergence", "SBDF1", "SMRK2"), ("Alternating", "CNAB2", "SMRK2"),
-                                                  ("Rotational", "CNRK2", "CNRK2"), ("SkewSymmetric", "SMRK2", "SMRK2")])
+                                                  ("Rotational", "CNRK2", "CNRK2"), ("SkewSymmetric", "SMRK2", "SMRK2"),
+                                                  ("SkewSymmetric", "CNFE1", "SMRK2")])
 def test_dns_schemes(gpu, method, stepping, init):
     flags = dict(C1_FLAGS, nonlinearity=method, timestepping=stepping, initstepping=init)
     u0 = random_field(24, 25, 24, LX_C, LZ_C, norm=0.1, decay=0.3, seed=2)
