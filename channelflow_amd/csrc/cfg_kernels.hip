@@ -2570,38 +2570,50 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
     const int f0 = c == 0 ? 3 : (c == 1 ? 4 : 5), f2 = c == 0 ? 5 : (c == 1 ? 7 : 8);
     const int f1 = XTERM ? (c == 0 ? 4 : (c == 1 ? 6 : 7)) : 3 + c;
     const long rowst = 2 * M;
-    const double* __restrict__ pg = fo + 2 * ((long)c * Ny * M + mcol) + part;
-    const double* __restrict__ p0 = fo + 2 * ((long)f0 * Ny * M + mcol) + (1 - part);
-    const double* __restrict__ p1 = fo + 2 * ((long)f1 * Ny * M + mcol) + part;
-    const double* __restrict__ p2 = fo + 2 * ((long)f2 * Ny * M + mcol) + (1 - part);
+    // columns past M read mode M - 1 (every field of fo has Ny x M modes); the
+    // values are masked where they are used
+    const long mcl = mok ? mcol : M - 1;
+    const double* __restrict__ pg = fo + 2 * ((long)c * Ny * M + mcl) + part;
+    const double* __restrict__ p0 = fo + 2 * ((long)f0 * Ny * M + mcl) + (1 - part);
+    const double* __restrict__ p1 = fo + 2 * ((long)f1 * Ny * M + mcl) + part;
+    const double* __restrict__ p2 = fo + 2 * ((long)f2 * Ny * M + mcl) + (1 - part);
     mfma4d ae[NRT], ao[NRT];
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt) ae[rt] = ao[rt] = mfma4d{0.0, 0.0, 0.0, 0.0};
     double dlast = 0.0;
-    auto gval = [&](int y) {
-        double G = have_g ? pg[y * rowst] : 0.0;
-        if (DIVF && XTERM) G += ZTERM ? sdx * p0[y * rowst] + sdz * p2[y * rowst] : sdx * p0[y * rowst];
-        return G;
-    };
-    double sgn[NS], dgn[NS], shn[NS], dhn[NS];
+    // Raw values of a chunk, per step: g at the upper point and its mirror,
+    // then (divergence forms) uu_c1, then the x and z terms of G. They are
+    // loaded without branches, at clamped addresses, and combined only when
+    // the chunk is multiplied: a load under a lane condition whose value is
+    // used at once gets its own wait, and the chunk's loads were then
+    // serialized instead of in flight behind the previous chunk's products.
+    constexpr int NR = DIVF ? 8 : 2;
+    double rv[NS][NR];
     auto fetch = [&](int y0) {
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            const int y = y0 + 4 * s + lk, ymr = Ny - 1 - y;
-            double G = 0, Gm = 0, H = 0, Hm = 0;
-            if (mok && y < nh) {
-                G = gval(y);
-                if (DIVF) H = beta * p1[y * rowst];
-                if (ymr != y) {
-                    Gm = gval(ymr);
-                    if (DIVF) Hm = beta * p1[ymr * rowst];
+            const int y = min(y0 + 4 * s + lk, nh - 1), ymr = Ny - 1 - y;
+            rv[s][0] = pg[y * rowst];
+            rv[s][1] = pg[ymr * rowst];
+            if (DIVF) {
+                rv[s][2] = p1[y * rowst];
+                rv[s][3] = p1[ymr * rowst];
+                if (XTERM) {
+                    rv[s][4] = p0[y * rowst];
+                    rv[s][5] = p0[ymr * rowst];
+                }
+                if (ZTERM) {
+                    rv[s][6] = p2[y * rowst];
+                    rv[s][7] = p2[ymr * rowst];
                 }
             }
-            sgn[s] = G + Gm;
-            dgn[s] = G - Gm;
-            shn[s] = H + Hm;
-            dhn[s] = H - Hm;
         }
+    };
+    // G at a point from its raw values (i: 0 the upper point, 1 its mirror)
+    auto gval = [&](const double* r, int i) {
+        double G = have_g ? r[i] : 0.0;
+        if (DIVF && XTERM) G += ZTERM ? sdx * r[4 + i] + sdz * r[6 + i] : sdx * r[4 + i];
+        return G;
     };
     // matrix entries of chunk y0: entry i = (matrix a, row slot, column kk)
     double mv[NST];
@@ -2622,10 +2634,20 @@ __global__ void __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(yw_wav
         double sg[NS], dg[NS], sh[NS], dh[NS];
 #pragma unroll
         for (int s = 0; s < NS; ++s) {
-            sg[s] = sgn[s];
-            dg[s] = dgn[s];
-            sh[s] = shn[s];
-            dh[s] = dhn[s];
+            const int y = y0 + 4 * s + lk, ymr = Ny - 1 - y;
+            double G = 0, Gm = 0, H = 0, Hm = 0;
+            if (mok && y < nh) {
+                G = gval(rv[s], 0);
+                if (DIVF) H = beta * rv[s][2];
+                if (ymr != y) {
+                    Gm = gval(rv[s], 1);
+                    if (DIVF) Hm = beta * rv[s][3];
+                }
+            }
+            sg[s] = G + Gm;
+            dg[s] = G - Gm;
+            sh[s] = H + Hm;
+            dh[s] = H - Hm;
         }
         // DB: buffer buf was last read two chunks ago, before every thread
         // passed the previous chunk's barrier
