@@ -1958,17 +1958,27 @@ __global__ void __launch_bounds__(256) k_xwinv(Geom g, const double2* __restrict
     const int fs = deriv ? fo - nsrc : fo;
     const double2* __restrict__ src = yp0 + bid.z * sy + (((long)fs * Ny + y) * nkz + (line_ok ? ikz : 0)) * nkx;
     const double dk = 2 * PI / g.Lx;
+    // All of a lane's loads are issued before any is used (bins it does not
+    // hold read bin 0 and are zeroed after): a load under the lane condition
+    // whose value was scaled at once got its own wait, one HBM round trip per
+    // load.
     double2 v[N2];
+    int ikr[N2];
 #pragma unroll
     for (int r = 0; r < N2; ++r) {
         const int n = j + N1 * r;
         int ikx = -1;
         if (n <= g.kxd) ikx = n;
         else if (n >= NX - g.kxd) ikx = n - NX + nkx;
+        ikr[r] = (line_ok && j < N1) ? ikx : -1;
+        v[r] = src[ikr[r] >= 0 ? ikr[r] : 0];
+    }
+#pragma unroll
+    for (int r = 0; r < N2; ++r) {
         double2 z = make_double2(0.0, 0.0);
-        if (line_ok && j < N1 && ikx >= 0) {
-            z = src[ikx];
-            if (deriv) z = imul(dk * kx_of(g, ikx), z);
+        if (ikr[r] >= 0) {
+            z = v[r];
+            if (deriv) z = imul(dk * kx_of(g, ikr[r]), z);
         }
         v[r] = z;
     }
