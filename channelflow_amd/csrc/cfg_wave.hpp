@@ -60,13 +60,24 @@ __device__ __forceinline__ double dpp_row(double v) {
     const int hi = __builtin_amdgcn_mov_dpp((int)(x >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
+// The same move with 1.0 in the lanes whose source lies outside the row: the
+// low word of 1.0 is 0 (zero fill), the high word is kept from a register
+// preset to 0x3FF00000.
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_one(double v) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0x3FF00000, (int)(x >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 constexpr int DPP_ROW_SHL = 0x100, DPP_ROW_SHR = 0x110;      // + shift 1..15
 constexpr int DPP_QUAD_XOR1 = 0xB1, DPP_QUAD_XOR2 = 0x4E;    // quad_perm [1,0,3,2], [2,3,0,1]
 constexpr int DPP_ROW_MIRROR = 0x140, DPP_ROW_HALF_MIRROR = 0x141;
 
 // v of segment lane + d (wdn) / lane - d (wup). A lane whose source is outside
-// the segment gets its own v (0 in 16-lane segments, which are DPP rows);
-// every caller discards that value.
+// the segment gets 0 in 16-lane segments, which are DPP rows, and the scans
+// there use that 0 (it adds nothing); in wider segments it gets its own v,
+// which every caller discards.
 template <int W>
 __device__ __forceinline__ double wdn(double v, int d) {
     if constexpr (W == 16) {
@@ -93,6 +104,39 @@ __device__ __forceinline__ double wup(double v, int d) {
     }
     return __shfl_up(v, d, W);
 }
+// wdn / wup for the multiplier of an affine scan: in 16-lane segments a lane
+// whose source is outside the segment gets 1.0, which with the 0 of the
+// constant term is the identity map, so those scan steps need no lane
+// condition (x + a * 0 = x and a * 1 = a). Wider segments as wdn / wup.
+template <int W>
+__device__ __forceinline__ double wdn1(double v, int d) {
+    if constexpr (W == 16) {
+        switch (d) {
+            case 1: return dpp_row_one<DPP_ROW_SHL + 1>(v);
+            case 2: return dpp_row_one<DPP_ROW_SHL + 2>(v);
+            case 4: return dpp_row_one<DPP_ROW_SHL + 4>(v);
+            case 8: return dpp_row_one<DPP_ROW_SHL + 8>(v);
+            default: break;
+        }
+    }
+    return __shfl_down(v, d, W);
+}
+template <int W>
+__device__ __forceinline__ double wup1(double v, int d) {
+    if constexpr (W == 16) {
+        switch (d) {
+            case 1: return dpp_row_one<DPP_ROW_SHR + 1>(v);
+            case 2: return dpp_row_one<DPP_ROW_SHR + 2>(v);
+            case 4: return dpp_row_one<DPP_ROW_SHR + 4>(v);
+            case 8: return dpp_row_one<DPP_ROW_SHR + 8>(v);
+            default: break;
+        }
+    }
+    return __shfl_up(v, d, W);
+}
+// DPP rows: lanes past a segment's ends read 0 (wdn, wup) or 1 (wdn1, wup1)
+template <int W>
+constexpr bool wrow() { return W == 16; }
 template <int W>
 __device__ __forceinline__ double wbc(double v, int src) { return __shfl(v, src, W); }
 
@@ -205,10 +249,10 @@ __device__ __forceinline__ void wsuffix(const double (&t)[K], double (&incl)[K],
     double x = loc[0];
     for (int d = 1; d < W; d <<= 1) {
         const double y = wdn<W>(x, d);
-        if (lane + d < W) x += y;
+        if (wrow<W>() || lane + d < W) x += y;
     }
     double after = wdn<W>(x, 1);
-    if (lane == W - 1) after = 0.0;
+    if (!wrow<W>() && lane == W - 1) after = 0.0;
     for (int k = 0; k < K; ++k) {
         incl[k] = loc[k] + after;
         excl[k] = (k + 1 < K) ? loc[k + 1] + after : after;
@@ -263,14 +307,14 @@ __device__ __forceinline__ void waff_suffix(double (&cc)[K], const double (&aa)[
         Fa[k] = xa;
     }
     for (int d = 1; d < W; d <<= 1) {
-        const double yc = wdn<W>(xc, d), ya = wdn<W>(xa, d);
-        if (lane + d < W) {
+        const double yc = wdn<W>(xc, d), ya = wdn1<W>(xa, d);
+        if (wrow<W>() || lane + d < W) {
             xc = xc + xa * yc;
             xa = xa * ya;
         }
     }
     double ec = wdn<W>(xc, 1);
-    if (lane == W - 1) ec = 0.0;
+    if (!wrow<W>() && lane == W - 1) ec = 0.0;
     for (int k = 0; k < K; ++k) cc[k] = Fc[k] + Fa[k] * ec;
 }
 
@@ -286,14 +330,14 @@ __device__ __forceinline__ void waff_prefix(double (&cc)[K], const double (&aa)[
         Ga[k] = xa;
     }
     for (int d = 1; d < W; d <<= 1) {
-        const double yc = wup<W>(xc, d), ya = wup<W>(xa, d);
-        if (lane >= d) {
+        const double yc = wup<W>(xc, d), ya = wup1<W>(xa, d);
+        if (wrow<W>() || lane >= d) {
             xc = xc + xa * yc;
             xa = xa * ya;
         }
     }
     double ec = wup<W>(xc, 1);
-    if (lane == 0) ec = 0.0;
+    if (!wrow<W>() && lane == 0) ec = 0.0;
     for (int k = 0; k < K; ++k) cc[k] = Gc[k] + Ga[k] * ec;
 }
 
@@ -325,8 +369,8 @@ __device__ __forceinline__ void waff_suffix_n(double (&cc)[NR][K], const double 
         double yc[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) yc[r] = wdn<W>(xc[r], d);
-        const double ya = wdn<W>(xa, d);
-        if (lane + d < W) {
+        const double ya = wdn1<W>(xa, d);
+        if (wrow<W>() || lane + d < W) {
 #pragma unroll
             for (int r = 0; r < NR; ++r) xc[r] = xc[r] + xa * yc[r];
             xa = xa * ya;
@@ -335,7 +379,7 @@ __device__ __forceinline__ void waff_suffix_n(double (&cc)[NR][K], const double 
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         double ec = wdn<W>(xc[r], 1);
-        if (lane == W - 1) ec = 0.0;
+        if (!wrow<W>() && lane == W - 1) ec = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) cc[r][k] = Fc[r][k] + Fa[k] * ec;
     }
@@ -362,8 +406,8 @@ __device__ __forceinline__ void waff_prefix_n(double (&cc)[NR][K], const double 
         double yc[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) yc[r] = wup<W>(xc[r], d);
-        const double ya = wup<W>(xa, d);
-        if (lane >= d) {
+        const double ya = wup1<W>(xa, d);
+        if (wrow<W>() || lane >= d) {
 #pragma unroll
             for (int r = 0; r < NR; ++r) xc[r] = xc[r] + xa * yc[r];
             xa = xa * ya;
@@ -372,7 +416,7 @@ __device__ __forceinline__ void waff_prefix_n(double (&cc)[NR][K], const double 
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         double ec = wup<W>(xc[r], 1);
-        if (lane == 0) ec = 0.0;
+        if (!wrow<W>() && lane == 0) ec = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) cc[r][k] = Gc[r][k] + Ga[k] * ec;
     }
@@ -423,8 +467,8 @@ __device__ __forceinline__ void whelm_par_n(const double (&f)[NR][K], const doub
         }
         const double lastp = wup<W>(f[r][K - 1], 1);
         const double firstn = wdn<W>(f[r][0], 1);
-        if (lane > 0) fm[0] = lastp;
-        fp[K - 1] = (lane < W - 1) ? firstn : 0.0;
+        if (wrow<W>() || lane > 0) fm[0] = lastp;
+        fp[K - 1] = (wrow<W>() || lane < W - 1) ? firstn : 0.0;
         // u = B f: zero past the system (zero rows), bc on row 0
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -569,7 +613,7 @@ __device__ __forceinline__ void wsuffix_n(const double (&t)[NR][K], double (&inc
         double y[NR];
 #pragma unroll
         for (int r = 0; r < NR; ++r) y[r] = wdn<W>(x[r], d);
-        if (lane + d < W) {
+        if (wrow<W>() || lane + d < W) {
 #pragma unroll
             for (int r = 0; r < NR; ++r) x[r] += y[r];
         }
@@ -577,7 +621,7 @@ __device__ __forceinline__ void wsuffix_n(const double (&t)[NR][K], double (&inc
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         double after = wdn<W>(x[r], 1);
-        if (lane == W - 1) after = 0.0;
+        if (!wrow<W>() && lane == W - 1) after = 0.0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             incl[r][k] = loc[r][k] + after;
