@@ -1999,10 +1999,15 @@ __global__ void __launch_bounds__(256) k_xwinv(Geom g, const double2* __restrict
 // field 6) and u_1u_2 (7) as fields 4 and 5. The y-forward transform then
 // reads f_c and u_c u_1 only. The added term is formed exactly as the
 // y-forward transform formed it before (the same products in the same order).
+// xcd: blocks in XCD order (xcd_grid), for grids whose rows of nkz bins are
+// not a whole number of 128-byte lines, so that neighbouring kz chunks, which
+// share the lines at their edges, read them through one L2; where the rows
+// are whole lines the dispatch order is faster.
 template <int NX>
 __global__ void __launch_bounds__(256) k_xwfwd(Geom g, const double2* __restrict__ twg,
                                                const double2* __restrict__ zo0, long sx, double2* __restrict__ fo0,
-                                               long sy, int fold) {
+                                               long sy, int fold, int xcd) {
+    const BlockId bid = xcd ? xcd_grid() : BlockId{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
     using S = ZW<NX>;
     constexpr int N1 = S::N1, N2 = S::N2;
     extern __shared__ double2 lds[];
@@ -2011,13 +2016,13 @@ __global__ void __launch_bounds__(256) k_xwfwd(Geom g, const double2* __restrict
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int nkz = g.nkz, nkx = g.nkx, Ny = g.Ny;
     const int nzc = (nkz + S::LPW - 1) / S::LPW;
-    const int wg = blockIdx.x * (blockDim.x / 64) + wave;
+    const int wg = bid.x * (blockDim.x / 64) + wave;
     const int y = wg / nzc;
     if (y >= Ny) return;
     const int L = lane / S::G, j = lane % S::G;
     const int ikz = (wg - y * nzc) * S::LPW + L;
     const bool line_ok = ikz < nkz;
-    const int t = blockIdx.y;
+    const int t = bid.y;
     int fa = t, oa = t, fb = -1, ob = -1;
     if (fold) {
         fa = t < 3 ? t : t + 3;
@@ -2028,7 +2033,7 @@ __global__ void __launch_bounds__(256) k_xwfwd(Geom g, const double2* __restrict
     double2* const T = lds + NX + (wave * S::LPW + L) * S::LBUF;
     auto line = [&](int f, double2 (&v)[N2]) {
         const double2* __restrict__ in =
-            zo0 + blockIdx.z * sx + ((long)f * Ny + y) * NX * (long)nkz + (line_ok ? ikz : 0);
+            zo0 + bid.z * sx + ((long)f * Ny + y) * NX * (long)nkz + (line_ok ? ikz : 0);
         double2 u[N1];
 #pragma unroll
         for (int m = 0; m < N1; ++m)
@@ -2040,7 +2045,7 @@ __global__ void __launch_bounds__(256) k_xwfwd(Geom g, const double2* __restrict
     if (fb >= 0) line(fb, vb);
     if (line_ok && j < N1) {
         const double scale = 1.0 / ((double)g.Nx * g.Nz), beta = 0.5;
-        double2* __restrict__ base = fo0 + blockIdx.z * sy;
+        double2* __restrict__ base = fo0 + bid.z * sy;
         double2* __restrict__ outa = base + (((long)oa * Ny + y) * nkz + ikz) * nkx;
         double2* __restrict__ outb = base + (((long)(ob < 0 ? oa : ob) * Ny + y) * nkz + ikz) * nkx;
 #pragma unroll
@@ -2815,7 +2820,8 @@ static bool launch_x(bool inverse, const Geom& g, const FFTPlan& px, int nsrc, i
             if (fold && nf != 8) throw std::runtime_error("x-FFT fold: the skew-symmetric layout has 8 fields");
             const dim3 grid((unsigned)((waves + wb - 1) / wb), fold ? 5 : nf, n);
             allow_lds(k_xwfwd<NX>, lds);
-            k_xwfwd<NX><<<grid, 64 * wb, lds, st>>>(g, px.tw, in, sin, out, sout, fold ? 1 : 0);
+            const int xcd = (g.nkz * (int)sizeof(double2)) % 128 != 0;
+            k_xwfwd<NX><<<grid, 64 * wb, lds, st>>>(g, px.tw, in, sin, out, sout, fold ? 1 : 0, xcd);
         }
         return fold && !inverse;
     }
